@@ -1,0 +1,180 @@
+/* pdt_capi.h -- C ABI of the MI355X TV/TGV primal-dual solver library
+ * (libpdt_hip.so).
+ *
+ * This is the boundary below the toolbox's C++ classes and Python modules:
+ * plain C types, opaque handles, int status codes.  Each entry point names the
+ * reference interface it replaces (paths relative to the primal-dual-toolbox
+ * source tree).
+ *
+ * Conventions
+ *  - Arrays use the reference's iu layout: size {s0, s1[, s2]} with dim 0
+ *    fastest (element (x, y, c) at x + s0 * (y + s1 * c)).  A numpy array of
+ *    shape (s2, s1, s0) in C order has the same memory image.
+ *  - Complex data is interleaved (re, im) pairs of the real type.
+ *  - Every call is blocking: it finishes its device work before returning.
+ *    A handle is not thread-safe; distinct handles may be used concurrently.
+ *  - Status 0 is success; on failure pdt_last_error() returns the message
+ *    (thread-local), mirroring the IuException texts of the reference.
+ */
+#ifndef PDT_CAPI_H
+#define PDT_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDT_API __attribute__((visibility("default")))
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+  PDT_OK = 0,
+  PDT_ERR_ARG = 1,     /* invalid argument / null handle                     */
+  PDT_ERR_SIZE = 2,    /* IU_SIZE_CHECK failures (mricartesianoperator.cu:177-179) */
+  PDT_ERR_STATE = 3,   /* "No operator is set!", "No noisy data is set!",
+                          "There is no result yet!" (tgvmrioptimizer.cu:96-110,
+                          tgvoptimizerwithop.cu:41-47)                        */
+  PDT_ERR_CONST = 4,   /* constant count/type (operatorbase.h:242-256,276-287,493-502) */
+  PDT_ERR_HIP = 5,     /* HIP runtime error (IU_CUDA_CHECK)                   */
+  PDT_ERR_UNSUPPORTED = 6
+};
+
+/* ---- element types ---------------------------------------------------- */
+enum {
+  PDT_F32 = 0,  /* float    (iu float)   */
+  PDT_C64 = 1,  /* complex<float>  (iu float2)  */
+  PDT_F64 = 2,  /* double   (iu double)  */
+  PDT_C128 = 3  /* complex<double> (iu double2) */
+};
+
+/* ---- operators (src/operator) ----------------------------------------- */
+enum {
+  PDT_OP_CARTESIAN = 0,              /* MriCartesianOperator (mricartesianoperator.h:31-70)            */
+  PDT_OP_CARTESIAN_REMOVE_ROOS = 1,  /* MriCartesianRemoveROOSOperator (mricartesianoperator.h:79-118) */
+  PDT_OP_SAMPLING = 2                /* MriSamplingOperator (mrisamplingoperator.h)                     */
+};
+
+/* ---- solvers (src/optimizer) ------------------------------------------ */
+enum {
+  PDT_SOLVER_TV = 0,      /* TvOptimizer (tvoptimizer.h:63-121)                 */
+  PDT_SOLVER_TGV = 1,     /* TgvOptimizer (tgvoptimizer.h)                      */
+  PDT_SOLVER_TGV_MRI = 2, /* TgvMriOptimizer (tgvmrioptimizer.h:75-150)         */
+  PDT_SOLVER_TV_OP = 3,   /* TvOptimizerWithOp (tvoptimizerwithop.h)            */
+  PDT_SOLVER_TGV_OP = 4   /* TgvOptimizerWithOp (tgvoptimizerwithop.h)          */
+};
+
+typedef struct pdt_array pdt_array;
+typedef struct pdt_op pdt_op;
+typedef struct pdt_solver pdt_solver;
+
+/* Union of TvParameters / TgvParameters / TgvMriParameters
+ * (tvoptimizer.h:32-61, tgvoptimizer.h:34-67, tgvmrioptimizer.h:33-66) plus
+ * fields that have no reference counterpart (marked NEW). */
+typedef struct pdt_params {
+  double lambda;      /* data weight (TV/TGV/with-op); unused by TGV-MRI (fixed 1) */
+  double alpha0;      /* TGV second-order weight                               */
+  double alpha1;      /* TGV first-order weight                                */
+  double reduction;   /* TGV-MRI alpha continuation factor                     */
+  uint32_t max_iter;  /* the solvers run max_iter + 1 iterations               */
+  uint32_t check;     /* print interval when verbose                           */
+  double gap_tol;     /* NEW: stop once rel. primal-dual gap <= gap_tol (0 = off) */
+  uint32_t gap_every; /* NEW: evaluate the gap every gap_every iterations (0 = never) */
+  int32_t dual_fp16;  /* NEW: store TGV dual fields p, q as fp16 (TGV denoising)  */
+  int32_t use_graph;  /* NEW: replay iteration chunks through a hipGraph (default 1) */
+  double radius_u;    /* NEW: partial-gap ball radius for u (0 = 0.01 ||A^*f||)  */
+  double radius_v;    /* NEW: partial-gap ball radius for v (0 = sqrt(8) radius_u) */
+} pdt_params;
+
+typedef struct pdt_stats {
+  uint32_t iters;     /* iterations run by the last solve                      */
+  uint32_t n_gap;     /* gap evaluations in the last solve                     */
+  double primal;      /* last primal energy P                                  */
+  double dual;        /* last dual value D                                     */
+  double gap;         /* P - D                                                 */
+  double rel_gap;     /* (P - D) / |P|                                         */
+  double ms;          /* device time of the iteration loop (hipEvent), ms      */
+  int32_t converged;  /* 1 if gap_tol was reached                              */
+} pdt_stats;
+
+/* ---- library ------------------------------------------------------------ */
+PDT_API const char* pdt_last_error(void);
+PDT_API int pdt_version(void);
+/* Selects the HIP device for subsequent handle creation. */
+PDT_API int pdt_set_device(int device);
+
+/* ---- device arrays: replace iu::LinearDeviceMemory<T, N> ----------------- */
+PDT_API int pdt_array_create(int dtype, int ndim, const int64_t* size, pdt_array** out);
+/* Wraps existing device memory (not owned, e.g. a torch tensor). */
+PDT_API int pdt_array_wrap(int dtype, int ndim, const int64_t* size, void* device_ptr,
+                           pdt_array** out);
+PDT_API int pdt_array_upload(pdt_array* a, const void* host, size_t bytes);
+PDT_API int pdt_array_download(const pdt_array* a, void* host, size_t bytes);
+PDT_API int pdt_array_info(const pdt_array* a, int* dtype, int* ndim, int64_t* size3);
+PDT_API void* pdt_array_device_ptr(pdt_array* a);
+PDT_API int pdt_array_destroy(pdt_array* a);
+
+/* ---- operators: replace OperatorBase<In,Out> and the MRI operators -------- */
+/* dtype: PDT_C64 or PDT_C128 (the reference instantiates float2/double2,
+ * mricartesianoperator.cu:436-444). */
+PDT_API int pdt_op_create(int kind, int dtype, pdt_op** out);
+/* OperatorBase::addConstant (operatorbase.h:238-257).  Order matters:
+ * Cartesian ops take [0] coil sensitivities (complex, 3D {W, Hk, C}) and
+ * [1] mask (real, 2D {W, Hk}); the sampling op takes [0] mask. */
+PDT_API int pdt_op_add_constant(pdt_op* op, const void* host, int dtype, int ndim,
+                                const int64_t* size);
+PDT_API int pdt_op_num_constants(const pdt_op* op, int* have, int* required);
+/* OperatorBase::forward / adjoint (operatorbase.h:297-312): dst must be
+ * pre-allocated at the right size; sizes and constants are checked first. */
+PDT_API int pdt_op_forward(pdt_op* op, const pdt_array* src, pdt_array* dst);
+PDT_API int pdt_op_adjoint(pdt_op* op, const pdt_array* src, pdt_array* dst);
+/* getInputSize / getOutputSize (mricartesianoperator.cu:183-194,296-308). */
+PDT_API int pdt_op_input_size(pdt_op* op, int out_ndim, const int64_t* out_size,
+                              int* in_ndim, int64_t* in_size);
+PDT_API int pdt_op_output_size(pdt_op* op, int in_ndim, const int64_t* in_size,
+                               int* out_ndim, int64_t* out_size);
+/* adjointnessCheck (mricartesianoperator.cu:267-282): random u, p from the
+ * reference's seeded generator; returns <Au, p> and <u, A^*p>. */
+PDT_API int pdt_op_adjointness(pdt_op* op, double* lhs_re, double* lhs_im, double* rhs_re,
+                               double* rhs_im);
+PDT_API int pdt_op_name(const pdt_op* op, char* buf, size_t len);
+PDT_API int pdt_op_destroy(pdt_op* op);
+
+/* ---- solvers: replace Tv/Tgv/TgvMri/TvWithOp/TgvWithOp optimizers --------- */
+/* dtype of the primal image: PDT_F32/PDT_F64 (denoising only) or
+ * PDT_C64/PDT_C128. */
+PDT_API int pdt_solver_create(int kind, int dtype, pdt_solver** out);
+PDT_API int pdt_params_default(int kind, pdt_params* p);
+PDT_API int pdt_solver_get_params(const pdt_solver* s, pdt_params* p);
+PDT_API int pdt_solver_set_params(pdt_solver* s, const pdt_params* p);
+/* setOperator (tgvmrioptimizer.cu:56-60): shared ownership (ref-counted). */
+PDT_API int pdt_solver_set_operator(pdt_solver* s, pdt_op* op);
+/* setNoisyData (tgvmrioptimizer.cu:82-91): copies f to the solver and resets
+ * the dual variable r to 0. */
+PDT_API int pdt_solver_set_noisy_data(pdt_solver* s, const pdt_array* f);
+/* setInput0 (tgvmrioptimizer.cu:63-79): u0 copied to u and u_bar, duals reset
+ * to 0.  v0 (NEW, may be NULL) warm-starts v = v_bar ({W, H, 2} planes x, y);
+ * with v0 = NULL this is the reference behaviour. */
+PDT_API int pdt_solver_set_input0(pdt_solver* s, const pdt_array* u0, const pdt_array* v0);
+/* solve (tvoptimizer.cu:77-125 etc.): runs max_iter + 1 iterations, or stops
+ * early at gap_tol when the gap is enabled. */
+PDT_API int pdt_solver_solve(pdt_solver* s, int verbose);
+/* getResult: borrowed handle to u, valid until the next solve/set_input0 or
+ * destroy.  STATE error before the first solve (the with-op solvers throw
+ * "There is no result yet!", tvoptimizerwithop.cu:41-47). */
+PDT_API int pdt_solver_get_result(pdt_solver* s, pdt_array** u);
+/* NEW: v of the TGV solvers ({W, H, 2}), borrowed. */
+PDT_API int pdt_solver_get_v(pdt_solver* s, pdt_array** v);
+PDT_API int pdt_solver_get_stats(const pdt_solver* s, pdt_stats* st);
+PDT_API int pdt_solver_destroy(pdt_solver* s);
+
+/* NEW: solve many configured solvers (same kind and dtype) together, with one
+ * persistent work-list launch per half-iteration for the whole batch.  Each
+ * solver keeps its own parameters, sizes, coils and stopping. */
+PDT_API int pdt_batch_solve(pdt_solver** solvers, int count, int verbose);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PDT_CAPI_H */

@@ -1,0 +1,1354 @@
+// pdt_capi.cpp -- host side of libpdt_hip.so: device arrays, operators,
+// solvers and the batched iteration engine behind the C ABI of pdt_capi.h.
+//
+// The solver state lives in HBM for the whole solve; an iteration is 2 (TV),
+// 2 (TGV) or 3 (MRI) kernel launches over persistent work lists, replayed in
+// chunks through a hipGraph.  The only host/device traffic inside a solve is
+// one small control/statistics copy per chunk.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "pdt_capi.h"
+#include "pdt_desc.hpp"
+#include "pdt_launch.hpp"
+#include "pdt_tune.hpp"
+
+using namespace pdt;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+namespace {
+thread_local std::string g_err;
+
+struct Err {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Err{code, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    std::ostringstream ss;
+    ss << what << ": " << hipGetErrorString(e);
+    fail(PDT_ERR_HIP, ss.str());
+  }
+}
+#define HIPC(x) hip_check((x), #x)
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return PDT_OK;
+  } catch (const Err& e) {
+    g_err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return PDT_ERR_ARG;
+  }
+}
+
+size_t esize(int dt) {
+  switch (dt) {
+    case PDT_F32: return 4;
+    case PDT_C64: return 8;
+    case PDT_F64: return 8;
+    case PDT_C128: return 16;
+  }
+  fail(PDT_ERR_ARG, "unknown dtype");
+}
+bool is_cplx(int dt) { return dt == PDT_C64 || dt == PDT_C128; }
+bool is_dbl(int dt) { return dt == PDT_F64 || dt == PDT_C128; }
+int real_of(int dt) { return is_dbl(dt) ? PDT_F64 : PDT_F32; }
+int cplx_of(int dt) { return is_dbl(dt) ? PDT_C128 : PDT_C64; }
+const char* dt_name(int dt) {
+  switch (dt) {
+    case PDT_F32: return "float";
+    case PDT_C64: return "float2";
+    case PDT_F64: return "double";
+    case PDT_C128: return "double2";
+  }
+  return "?";
+}
+
+// RAII device buffer
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t b) { alloc(b); }
+  void alloc(size_t b) {
+    release();
+    if (b) HIPC(hipMalloc(&p, b));
+    bytes = b;
+  }
+  void ensure(size_t b) {
+    if (b > bytes) alloc(b);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Twiddle tables, one per (device, length, precision), kept for the process.
+struct TwKey {
+  int dev, n, dbl;
+  bool operator<(const TwKey& o) const {
+    return dev != o.dev ? dev < o.dev : n != o.n ? n < o.n : dbl < o.dbl;
+  }
+};
+std::mutex g_tw_mu;
+std::map<TwKey, void*> g_tw;
+
+std::vector<int> factorize(int n) {
+  std::vector<int> f;
+  while (n % 8 == 0 && n > 8) { f.push_back(8); n /= 8; }
+  for (int p : {8, 4, 2, 3, 5}) while (n % p == 0 && n >= p) { f.push_back(p); n /= p; }
+  for (int p = 7; n > 1; p += 2) while (n % p == 0) { f.push_back(p); n /= p; }
+  return f;
+}
+
+FftPlanD make_plan(int n, bool dbl) {
+  FftPlanD pl{};
+  pl.n = n;
+  std::vector<int> f = n > 1 ? factorize(n) : std::vector<int>{};
+  if ((int)f.size() > kMaxStages) fail(PDT_ERR_UNSUPPORTED, "FFT length has too many factors");
+  for (int p : f)
+    if (p > 64) fail(PDT_ERR_UNSUPPORTED, "FFT length has a prime factor above 64: " + std::to_string(n));
+  pl.ns = (int)f.size();
+  for (size_t i = 0; i < f.size(); ++i) pl.r[i] = f[i];
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  TwKey key{dev, n, dbl ? 1 : 0};
+  auto it = g_tw.find(key);
+  if (it == g_tw.end()) {
+    const long double twopi = 6.283185307179586476925286766559L;
+    void* d = nullptr;
+    if (dbl) {
+      std::vector<double> h(2 * size_t(n));
+      for (int m = 0; m < n; ++m) {
+        long double a = -twopi * m / n;
+        h[2 * m] = double(cosl(a));
+        h[2 * m + 1] = double(sinl(a));
+      }
+      HIPC(hipMalloc(&d, h.size() * sizeof(double)));
+      HIPC(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> h(2 * size_t(n));
+      for (int m = 0; m < n; ++m) {
+        long double a = -twopi * m / n;
+        h[2 * m] = float(cosl(a));
+        h[2 * m + 1] = float(sinl(a));
+      }
+      HIPC(hipMalloc(&d, h.size() * sizeof(float)));
+      HIPC(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    it = g_tw.emplace(key, d).first;
+  }
+  pl.tw = it->second;
+  return pl;
+}
+
+constexpr size_t kLdsBudget = 64 * 1024;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// arrays
+// ---------------------------------------------------------------------------
+struct pdt_array {
+  int dtype = PDT_F32;
+  int ndim = 0;
+  int64_t size[3] = {1, 1, 1};
+  void* ptr = nullptr;
+  bool owned = false;
+  size_t numel() const { return size_t(size[0]) * size_t(size[1]) * size_t(size[2]); }
+  size_t bytes() const { return numel() * esize(dtype); }
+  ~pdt_array() {
+    if (owned && ptr) (void)hipFree(ptr);
+  }
+};
+
+namespace {
+pdt_array* new_array(int dtype, int ndim, const int64_t* size) {
+  if (ndim < 1 || ndim > 3) fail(PDT_ERR_ARG, "ndim must be 1..3");
+  auto a = std::make_unique<pdt_array>();
+  a->dtype = dtype;
+  a->ndim = ndim;
+  for (int i = 0; i < ndim; ++i) {
+    if (size[i] < 0) fail(PDT_ERR_ARG, "negative size");
+    a->size[i] = size[i];
+  }
+  (void)esize(dtype);
+  if (a->bytes()) HIPC(hipMalloc(&a->ptr, a->bytes()));
+  a->owned = true;
+  return a.release();
+}
+
+std::string size_str(const int64_t* s, int nd) {
+  std::ostringstream ss;
+  ss << "(";
+  for (int i = 0; i < nd; ++i) ss << (i ? ", " : "") << s[i];
+  ss << ")";
+  return ss.str();
+}
+
+// IU_SIZE_CHECK
+void size_check(const int64_t* a, int na, const int64_t* b, int nb) {
+  bool ok = na == nb;
+  for (int i = 0; ok && i < na; ++i) ok = a[i] == b[i];
+  if (!ok) fail(PDT_ERR_SIZE, "Size mismatch: " + size_str(a, na) + " vs " + size_str(b, nb));
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// operators
+// ---------------------------------------------------------------------------
+struct HostConst {
+  int dtype, ndim;
+  int64_t size[3];
+  std::unique_ptr<pdt_array> dev;
+  std::vector<char> host;
+};
+
+struct pdt_op {
+  std::atomic<int> refs{1};
+  int kind = PDT_OP_CARTESIAN;
+  int dtype = PDT_C64;
+  std::vector<HostConst> consts;
+  // prepared geometry (valid once prepared == true)
+  bool prepared = false;
+  int W = 0, Hk = 0, H = 0, C = 1, y0 = 0, ncols = 0;
+  DevBuf Mc, cols, colmap;
+  std::vector<int> h_cols;
+  FftPlanD planW{}, planH{};
+  // operator-mode scratch
+  DevBuf T1, T2, ctl, slice, rows, colitems;
+  int nrows = 0, ncolitems = 0, cb = 1, cg = 1;
+  size_t lds_rows = 0, lds_cols = 0;
+  hipStream_t stream = nullptr;
+
+  int required() const { return kind == PDT_OP_SAMPLING ? 1 : 2; }
+  const char* name() const {
+    switch (kind) {
+      case PDT_OP_CARTESIAN: return "MriCartesianOperator";
+      case PDT_OP_CARTESIAN_REMOVE_ROOS: return "MriCartesianRemoveROOSOperator";
+      default: return "MriSamplingOperator";
+    }
+  }
+  std::string prefix() const { return std::string("Operator '") + name() + "': "; }
+
+  // OperatorBase::getConstant type check (operatorbase.h:263-290)
+  const HostConst& get_const(unsigned idx, int dtype_want, int ndim_want) const {
+    if (idx >= consts.size()) {
+      std::ostringstream ss;
+      ss << prefix() << "Requested index (" << idx << ") is not valid. Constants available: " << consts.size();
+      fail(PDT_ERR_CONST, ss.str());
+    }
+    const HostConst& c = consts[idx];
+    if (c.dtype != dtype_want || c.ndim != ndim_want) {
+      std::ostringstream ss;
+      ss << prefix() << "Data at idx (" << idx
+         << ") cannot be cast to type iu::LinearDeviceMemory<PixelType, Ndim> with PixelType="
+         << dt_name(dtype_want) << " Ndim=" << ndim_want;
+      fail(PDT_ERR_CONST, ss.str());
+    }
+    return c;
+  }
+
+  // OperatorBase::operatorCheck constant-count part (operatorbase.h:491-503)
+  void check_count() const {
+    if ((int)consts.size() != required()) {
+      std::ostringstream ss;
+      ss << prefix() << "Number of constants (" << consts.size()
+         << ") does not match number of required constants (" << required() << ")";
+      fail(PDT_ERR_CONST, ss.str());
+    }
+  }
+
+  const HostConst& mask() const {
+    return get_const(kind == PDT_OP_SAMPLING ? 0 : 1, real_of(dtype), 2);
+  }
+  const HostConst* sens() const {
+    return kind == PDT_OP_SAMPLING ? nullptr : &get_const(0, cplx_of(dtype), 3);
+  }
+  const void* S_dev() const {
+    const HostConst* s = sens();
+    return s ? s->dev->ptr : nullptr;
+  }
+
+  void prepare() {
+    check_count();
+    if (prepared) return;
+    const HostConst& m = mask();
+    const HostConst* s = sens();
+    W = int(m.size[0]);
+    Hk = int(m.size[1]);
+    if (s) {
+      C = int(s->size[2]);
+      const int64_t want[2] = {s->size[0], s->size[1]};
+      size_check(m.size, 2, want, 2);
+    } else {
+      C = 1;
+    }
+    if (kind == PDT_OP_CARTESIAN_REMOVE_ROOS) {
+      H = Hk / 2;
+      y0 = int((unsigned)(float(Hk) * 0.25f + 1));  // mricartesianoperator.cu:341
+      if (y0 + H > Hk) fail(PDT_ERR_SIZE, prefix() + "read-out window exceeds the k-space grid");
+    } else {
+      H = Hk;
+      y0 = 0;
+    }
+    if (W > kMaxRowLen) fail(PDT_ERR_UNSUPPORTED, prefix() + "PE length above 1024 is not supported");
+    // listed columns: PE lines with any nonzero mask entry
+    const bool dbl = is_dbl(dtype);
+    auto mval = [&](int kx, int kfe) -> double {
+      size_t i = size_t(kfe) * W + kx;
+      return dbl ? reinterpret_cast<const double*>(m.host.data())[i]
+                 : double(reinterpret_cast<const float*>(m.host.data())[i]);
+    };
+    h_cols.clear();
+    std::vector<int> cmap(W, -1);
+    for (int kx = 0; kx < W; ++kx) {
+      bool any = false;
+      for (int kfe = 0; kfe < Hk && !any; ++kfe) any = mval(kx, kfe) != 0.0;
+      if (any) {
+        cmap[kx] = (int)h_cols.size();
+        h_cols.push_back(kx);
+      }
+    }
+    ncols = (int)h_cols.size();
+    const size_t re = dbl ? 8 : 4;
+    std::vector<char> mc(std::max<size_t>(1, size_t(ncols) * Hk * re));
+    for (int ks = 0; ks < ncols; ++ks)
+      for (int kfe = 0; kfe < Hk; ++kfe) {
+        double v = mval(h_cols[ks], kfe);
+        if (dbl) reinterpret_cast<double*>(mc.data())[size_t(ks) * Hk + kfe] = v;
+        else reinterpret_cast<float*>(mc.data())[size_t(ks) * Hk + kfe] = float(v);
+      }
+    Mc.alloc(mc.size());
+    HIPC(hipMemcpy(Mc.p, mc.data(), mc.size(), hipMemcpyHostToDevice));
+    cols.alloc(std::max<size_t>(1, h_cols.size()) * sizeof(int));
+    if (ncols) HIPC(hipMemcpy(cols.p, h_cols.data(), h_cols.size() * sizeof(int), hipMemcpyHostToDevice));
+    colmap.alloc(size_t(W) * sizeof(int));
+    HIPC(hipMemcpy(colmap.p, cmap.data(), cmap.size() * sizeof(int), hipMemcpyHostToDevice));
+    planW = make_plan(W, dbl);
+    planH = make_plan(Hk, dbl);
+    prepared = true;
+  }
+
+  // LDS chunking shared by the operator mode and the solvers
+  void choose_tiles(int& cb_out, int& cg_out, size_t& lr, size_t& lc) const {
+    const int dt = dtype == PDT_C128 ? DT_C128 : DT_C64;
+    const size_t e = is_dbl(dtype) ? 16 : 8;
+    int cbm = int(kLdsBudget / (2 * size_t(W) * e));
+    cb_out = std::max(1, std::min(C, cbm));
+    int cgm = int(kLdsBudget / (2 * size_t(Hk) * e));
+    cg_out = std::max(1, std::min({4, std::max(ncols, 1), cgm}));
+    if (cgm < 1) fail(PDT_ERR_UNSUPPORTED, prefix() + "FE length too large for the LDS column pass");
+    lr = lds_rows_bytes(dt, cb_out, W);
+    lc = lds_cols_bytes(dt, cg_out, Hk);
+  }
+
+  void ensure_opmode() {
+    prepare();
+    if (!stream) HIPC(hipStreamCreate(&stream));
+    const size_t ce = esize(cplx_of(dtype));
+    T1.ensure(std::max<size_t>(1, size_t(C) * H * ncols * ce));
+    T2.ensure(std::max<size_t>(1, size_t(C) * H * ncols * ce));
+    if (!ctl.p) {
+      ctl.alloc(CTL_WORDS * sizeof(int));
+      int h[CTL_WORDS] = {1, 0, 0, 0, 0, 0, 0, 0};
+      HIPC(hipMemcpy(ctl.p, h, sizeof(h), hipMemcpyHostToDevice));
+    }
+    choose_tiles(cb, cg, lds_rows, lds_cols);
+    if (!rows.p) {
+      std::vector<int2> r(H);
+      for (int y = 0; y < H; ++y) r[y] = make_int2(0, y);
+      rows.alloc(r.size() * sizeof(int2));
+      HIPC(hipMemcpy(rows.p, r.data(), r.size() * sizeof(int2), hipMemcpyHostToDevice));
+      nrows = H;
+      const int ng = (ncols + cg - 1) / cg;
+      std::vector<int2> c;
+      for (int cc = 0; cc < C; ++cc)
+        for (int g = 0; g < ng; ++g) c.push_back(make_int2(0, cc * ng + g));
+      colitems.alloc(std::max<size_t>(1, c.size()) * sizeof(int2));
+      if (!c.empty()) HIPC(hipMemcpy(colitems.p, c.data(), c.size() * sizeof(int2), hipMemcpyHostToDevice));
+      ncolitems = (int)c.size();
+      slice.alloc(sizeof(SliceDev));
+    }
+  }
+
+  SliceDev geometry() const {
+    SliceDev d{};
+    d.W = W;
+    d.H = H;
+    d.Hk = Hk;
+    d.y0 = y0;
+    d.C = C;
+    d.ncols = ncols;
+    d.S = S_dev();
+    d.Mc = Mc.p;
+    d.cols = cols.as<int>();
+    d.colmap = colmap.as<int>();
+    d.planW = planW;
+    d.planH = planH;
+    return d;
+  }
+
+  // Applies A (forward) or A^* (adjoint) from device src to device dst.
+  void apply(bool forward, const void* src, void* dst, hipStream_t st) {
+    ensure_opmode();
+    SliceDev d = geometry();
+    d.ngroups = (ncols + cg - 1) / cg;
+    d.T1 = T1.p;
+    d.T2 = T2.p;
+    d.op_in = src;
+    d.op_out = dst;
+    d.ctl = ctl.as<int>();
+    HIPC(hipMemcpyAsync(slice.p, &d, sizeof(d), hipMemcpyHostToDevice, st));
+    Launch L;
+    L.stream = st;
+    L.sl = slice.as<SliceDev>();
+    L.nsl = 1;
+    L.rows = rows.as<int2>();
+    L.nrows = nrows;
+    L.cols = colitems.as<int2>();
+    L.ncolitems = ncolitems;
+    L.cb = cb;
+    L.cg = cg;
+    L.lds_rows = lds_rows;
+    L.lds_cols = lds_cols;
+    const int dt = dtype == PDT_C128 ? DT_C128 : DT_C64;
+    if (forward) {
+      run_zero(st, dst, size_t(W) * Hk * C * esize(cplx_of(dtype)));
+      if (ncols) run_op_forward(L, dt);
+    } else {
+      if (ncols) run_op_adjoint(L, dt);
+      else run_zero(st, dst, size_t(W) * H * esize(cplx_of(dtype)));
+    }
+    HIPC(hipGetLastError());
+  }
+
+  ~pdt_op() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+void op_release(pdt_op* op) {
+  if (op && --op->refs == 0) delete op;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// solvers
+// ---------------------------------------------------------------------------
+struct pdt_solver {
+  int kind = PDT_SOLVER_TV;
+  int dtype = PDT_F32;
+  pdt_params prm{};
+  pdt_op* op = nullptr;
+  // data
+  std::unique_ptr<pdt_array> f;  // copy of the noisy data
+  bool data_dirty = true;         // compact data / r must be rebuilt
+  // state
+  int W = 0, H = 0;
+  bool have_u = false;
+  std::unique_ptr<pdt_array> u, v;  // result handles (u: {W,H}, v: {W,H,2})
+  DevBuf ub, vb, p, q, tmp;
+  DevBuf fk, rk, T1, T2;
+  int Hk = 0, C = 0, ncols = 0;
+  double F0 = 0, atf_norm = 0;
+  int rupd = 0;  // r updates since set_noisy_data
+  pdt_stats stats{};
+  hipStream_t stream = nullptr;
+
+  bool is_op() const { return kind == PDT_SOLVER_TGV_MRI || kind == PDT_SOLVER_TV_OP || kind == PDT_SOLVER_TGV_OP; }
+  bool is_tgv() const { return kind == PDT_SOLVER_TGV || kind == PDT_SOLVER_TGV_MRI || kind == PDT_SOLVER_TGV_OP; }
+  size_t pe() const { return esize(dtype); }
+  size_t pq_e() const { return (prm.dual_fp16 && kind == PDT_SOLVER_TGV && !is_dbl(dtype)) ? pe() / 2 : pe(); }
+
+  ~pdt_solver() {
+    op_release(op);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void default_params(int kind, pdt_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  // TvParameters / TgvParameters / TgvMriParameters constructors
+  // (tvoptimizer.cu:26-29, tgvoptimizer.cu:27-30, tgvmrioptimizer.cu:28-31)
+  p->lambda = 1;
+  p->alpha0 = 1;
+  p->alpha1 = 1;
+  p->reduction = 1;
+  p->max_iter = 1;
+  p->check = 100;
+  p->gap_tol = 0;
+  p->gap_every = 0;
+  p->dual_fp16 = 0;
+  p->use_graph = 1;
+  p->radius_u = 0;
+  p->radius_v = 0;
+  (void)kind;
+}
+
+void alloc_state(pdt_solver* s, int W, int H) {
+  const size_t n = size_t(W) * H, e = s->pe();
+  const int64_t us[2] = {W, H};
+  s->u.reset(new_array(s->dtype, 2, us));
+  s->ub.alloc(n * e);
+  s->p.alloc(2 * n * s->pq_e());
+  if (s->is_tgv()) {
+    const int64_t vs[3] = {W, H, 2};
+    s->v.reset(new_array(s->dtype, 3, vs));
+    s->vb.alloc(2 * n * e);
+    s->q.alloc(3 * n * s->pq_e());
+    if (s->kind == PDT_SOLVER_TGV) s->tmp.alloc(2 * n * e);
+  }
+  s->W = W;
+  s->H = H;
+}
+
+// setInput0 (tgvmrioptimizer.cu:63-79): u = u_bar = input, v = p = q = 0.
+void set_input0(pdt_solver* s, const void* u0_dev, const void* v0_dev, int W, int H) {
+  if (!s->stream) HIPC(hipStreamCreate(&s->stream));
+  alloc_state(s, W, H);
+  const size_t n = size_t(W) * H, e = s->pe();
+  HIPC(hipMemcpyAsync(s->u->ptr, u0_dev, n * e, hipMemcpyDeviceToDevice, s->stream));
+  HIPC(hipMemcpyAsync(s->ub.p, u0_dev, n * e, hipMemcpyDeviceToDevice, s->stream));
+  run_zero(s->stream, s->p.p, s->p.bytes);
+  if (s->is_tgv()) {
+    run_zero(s->stream, s->q.p, s->q.bytes);
+    if (v0_dev) {
+      HIPC(hipMemcpyAsync(s->v->ptr, v0_dev, 2 * n * e, hipMemcpyDeviceToDevice, s->stream));
+      HIPC(hipMemcpyAsync(s->vb.p, v0_dev, 2 * n * e, hipMemcpyDeviceToDevice, s->stream));
+    } else {
+      run_zero(s->stream, s->v->ptr, 2 * n * e);
+      run_zero(s->stream, s->vb.p, 2 * n * e);
+    }
+  }
+  HIPC(hipStreamSynchronize(s->stream));
+  s->have_u = true;
+}
+
+// Builds the compact data layout and r = 0 for an operator solver, and the
+// quantities of the partial gap (||A^* f||, ||f||^2 off the sampled lines).
+void prepare_op_data(pdt_solver* s) {
+  pdt_op* op = s->op;
+  op->prepare();
+  const int64_t want[3] = {op->W, op->Hk, op->C};
+  const int fnd = op->kind == PDT_OP_SAMPLING ? 2 : 3;
+  size_check(s->f->size, s->f->ndim, want, fnd);
+  if (!s->data_dirty && s->Hk == op->Hk && s->C == op->C && s->ncols == op->ncols) return;
+  if (!s->stream) HIPC(hipStreamCreate(&s->stream));
+  const size_t ce = s->pe();
+  s->Hk = op->Hk;
+  s->C = op->C;
+  s->ncols = op->ncols;
+  s->fk.alloc(std::max<size_t>(1, size_t(op->C) * op->ncols * op->Hk * ce));
+  s->rk.alloc(std::max<size_t>(1, size_t(op->C) * op->ncols * op->Hk * ce));
+  s->T1.alloc(std::max<size_t>(1, size_t(op->C) * op->H * op->ncols * ce));
+  s->T2.alloc(std::max<size_t>(1, size_t(op->C) * op->H * op->ncols * ce));
+  // compact gather + F0
+  SliceDev d = op->geometry();
+  d.fk = s->fk.p;
+  d.rk = s->rk.p;
+  DevBuf dsl(sizeof(SliceDev));
+  HIPC(hipMemcpy(dsl.p, &d, sizeof(d), hipMemcpyHostToDevice));
+  std::vector<int2> items;
+  for (int c = 0; c < op->C; ++c)
+    for (int kfe = 0; kfe < op->Hk; ++kfe) items.push_back(make_int2(0, c * op->Hk + kfe));
+  DevBuf ditems(items.size() * sizeof(int2));
+  HIPC(hipMemcpy(ditems.p, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+  DevBuf part(items.size() * kGapTerms * sizeof(double));
+  Launch L;
+  L.stream = s->stream;
+  L.sl = dsl.as<SliceDev>();
+  L.nsl = 1;
+  L.rows = ditems.as<int2>();
+  L.nrows = (int)items.size();
+  run_compact_data(L, is_dbl(s->dtype) ? DT_C128 : DT_C64, s->f->ptr, part.as<double>());
+  HIPC(hipGetLastError());
+  std::vector<double> hp(items.size() * kGapTerms);
+  HIPC(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPC(hipStreamSynchronize(s->stream));
+  double f0 = 0;
+  for (size_t i = 0; i < items.size(); ++i) f0 += hp[i * kGapTerms];
+  s->F0 = f0;
+  // ||A^* f|| (radius of the partial-gap ball)
+  const size_t n = size_t(op->W) * op->H;
+  DevBuf atf(n * ce);
+  op->apply(false, s->f->ptr, atf.p, s->stream);
+  std::vector<double> h(n * 2);
+  if (is_dbl(s->dtype)) {
+    HIPC(hipMemcpyAsync(h.data(), atf.p, n * ce, hipMemcpyDeviceToHost, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
+  } else {
+    std::vector<float> hf(n * 2);
+    HIPC(hipMemcpyAsync(hf.data(), atf.p, n * ce, hipMemcpyDeviceToHost, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < hf.size(); ++i) h[i] = hf[i];
+  }
+  double ss = 0;
+  for (double x : h) ss += x * x;
+  s->atf_norm = std::sqrt(ss);
+  s->rupd = 0;
+  s->data_dirty = false;
+}
+
+// ---------------------------------------------------------------------------
+// The engine: one or many slices, persistent work lists, graph-replayed chunks
+// ---------------------------------------------------------------------------
+struct Engine {
+  int kind = 0, dtype = 0;
+  bool fp16 = false;
+  std::vector<pdt_solver*> sv;
+  hipStream_t stream = nullptr;
+  DevBuf d_slices, d_rows, d_cols, d_part, d_ctl, d_stats;
+  int* h_ctl = nullptr;       // pinned mirrors
+  double* h_stats = nullptr;
+  int nrows = 0, ncolitems = 0;
+  Launch L;
+  std::vector<SliceDev> hs;
+  struct GraphEntry {
+    int len;
+    bool gap;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+
+  ~Engine() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    if (h_ctl) (void)hipHostFree(h_ctl);
+    if (h_stats) (void)hipHostFree(h_stats);
+  }
+
+  void build() {
+    pdt_solver* s0 = sv[0];
+    kind = s0->kind;
+    dtype = s0->dtype;
+    fp16 = s0->prm.dual_fp16 && kind == PDT_SOLVER_TGV && !is_dbl(dtype);
+    stream = s0->stream;
+    const int ns = (int)sv.size();
+    const bool mri = s0->is_op();
+    d_ctl.alloc(size_t(ns) * CTL_WORDS * sizeof(int));
+    d_stats.alloc(size_t(ns) * ST_WORDS * sizeof(double));
+    HIPC(hipHostMalloc((void**)&h_ctl, size_t(ns) * CTL_WORDS * sizeof(int), hipHostMallocDefault));
+    HIPC(hipHostMalloc((void**)&h_stats, size_t(ns) * ST_WORDS * sizeof(double), hipHostMallocDefault));
+    // tile choices (shared by the whole batch)
+    int cb = 64, cg = 4, rpi = 1;
+    size_t lr = 0, lc = 0;
+    int maxW = 0, maxHk = 0;
+    for (auto* s : sv) {
+      maxW = std::max(maxW, s->W);
+      if (mri) maxHk = std::max(maxHk, s->op->Hk);
+    }
+    const int dt = is_dbl(dtype) ? (is_cplx(dtype) ? DT_C128 : DT_F64) : (is_cplx(dtype) ? DT_C64 : DT_F32);
+    if (mri) {
+      const size_t e = is_dbl(dtype) ? 16 : 8;
+      int maxC = 1, maxNc = 1;
+      for (auto* s : sv) {
+        maxC = std::max(maxC, s->op->C);
+        maxNc = std::max(maxNc, s->op->ncols);
+      }
+      TuneKey key{kind, dt, maxW, sv[0]->H, maxHk, maxC, maxNc, ns};
+      TuneVal tv = tune_lookup(key);
+      cb = tv.cb > 0 ? tv.cb : maxC;
+      cg = tv.cg > 0 ? tv.cg : 4;
+      cb = std::max(1, std::min({cb, maxC, int(kLdsBudget / (2 * size_t(maxW) * e))}));
+      cg = std::max(1, std::min({cg, maxNc, int(kLdsBudget / (2 * size_t(maxHk) * e))}));
+      lr = lds_rows_bytes(dt, cb, maxW);
+      lc = lds_cols_bytes(dt, cg, maxHk);
+    } else {
+      TuneKey key{kind, dt, maxW, sv[0]->H, 0, 0, 0, ns};
+      TuneVal tv = tune_lookup(key);
+      rpi = tv.rpi > 0 ? tv.rpi : 1;
+    }
+    // slices and work lists
+    hs.assign(ns, SliceDev{});
+    std::vector<int2> rows, cols;
+    for (int i = 0; i < ns; ++i) {
+      pdt_solver* s = sv[i];
+      SliceDev d = mri ? s->op->geometry() : SliceDev{};
+      d.W = s->W;
+      d.H = s->H;
+      d.kind = s->kind;
+      d.dual_fp16 = fp16;
+      d.theta = 1.0;
+      d.alpha0 = s->prm.alpha0;
+      d.alpha1 = s->prm.alpha1;
+      d.alpha0_end = s->prm.alpha0 * s->prm.reduction;
+      d.alpha1_end = s->prm.alpha1 * s->prm.reduction;
+      d.kmax = double(s->prm.max_iter);
+      d.schedule = s->kind == PDT_SOLVER_TGV_MRI;
+      d.gap_tol = s->prm.gap_every ? s->prm.gap_tol : 0.0;
+      switch (s->kind) {
+        case PDT_SOLVER_TV:  // tvoptimizer.cu:98-100
+          d.tau = d.sigma = 1.0 / std::sqrt(8.0);
+          d.lambda = s->prm.lambda;
+          break;
+        case PDT_SOLVER_TGV:  // tgvoptimizer.cu:102-104
+          d.tau = d.sigma = 1.0 / std::sqrt(12.0);
+          d.lambda = s->prm.lambda;
+          break;
+        case PDT_SOLVER_TGV_MRI:  // tgvmrioptimizer.cu:132-133, lambda = 1 (:174)
+          d.tau = 1.0 / 16.0;
+          d.sigma = 1.0 / 8.0;
+          d.lambda = 1.0;
+          break;
+        default:  // with-operator solvers (tgvoptimizerwithop.cu:123-124)
+          d.tau = d.sigma = 1.0 / 8.0;
+          d.lambda = s->prm.lambda;
+          break;
+      }
+      if (s->kind == PDT_SOLVER_TV_OP) {
+        d.alpha0 = d.alpha1 = 1.0;
+        d.alpha0_end = d.alpha1_end = 1.0;
+      }
+      d.radius_u = s->prm.radius_u > 0 ? s->prm.radius_u : 0.01 * s->atf_norm;
+      d.radius_v = s->prm.radius_v > 0 ? s->prm.radius_v : std::sqrt(8.0) * d.radius_u;
+      d.F0 = s->F0;
+      d.u = s->u->ptr;
+      d.ub = s->ub.p;
+      d.p = s->p.p;
+      if (s->is_tgv()) {
+        d.v = s->v->ptr;
+        d.vb = s->vb.p;
+        d.q = s->q.p;
+        d.tmp = s->tmp.p;
+      }
+      d.f = s->f->ptr;
+      if (mri) {
+        d.fk = s->fk.p;
+        d.rk = s->rk.p;
+        d.T1 = s->T1.p;
+        d.T2 = s->T2.p;
+        d.ngroups = (d.ncols + cg - 1) / cg;
+      }
+      d.ctl = d_ctl.as<int>() + size_t(i) * CTL_WORDS;
+      d.stats = d_stats.as<double>() + size_t(i) * ST_WORDS;
+      d.row_beg = (int)rows.size();
+      const int step = mri ? 1 : rpi;
+      for (int y = 0; y < s->H; y += step) rows.push_back(make_int2(i, y));
+      d.row_end = (int)rows.size();
+      d.col_beg = (int)cols.size();
+      if (mri)
+        for (int c = 0; c < d.C; ++c)
+          for (int g = 0; g < d.ngroups; ++g) cols.push_back(make_int2(i, c * d.ngroups + g));
+      d.col_end = (int)cols.size();
+      hs[i] = d;
+    }
+    nrows = (int)rows.size();
+    ncolitems = (int)cols.size();
+    d_slices.alloc(hs.size() * sizeof(SliceDev));
+    HIPC(hipMemcpy(d_slices.p, hs.data(), hs.size() * sizeof(SliceDev), hipMemcpyHostToDevice));
+    d_rows.alloc(std::max<size_t>(1, rows.size()) * sizeof(int2));
+    HIPC(hipMemcpy(d_rows.p, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
+    d_cols.alloc(std::max<size_t>(1, cols.size()) * sizeof(int2));
+    if (!cols.empty()) HIPC(hipMemcpy(d_cols.p, cols.data(), cols.size() * sizeof(int2), hipMemcpyHostToDevice));
+    const size_t nitem = std::max<size_t>(1, std::max(rows.size(), cols.size()));
+    d_part.alloc(4 * nitem * kGapTerms * sizeof(double));
+    L.stream = stream;
+    L.sl = d_slices.as<SliceDev>();
+    L.nsl = ns;
+    L.rows = d_rows.as<int2>();
+    L.nrows = nrows;
+    L.rpi = rpi;
+    L.cols = d_cols.as<int2>();
+    L.ncolitems = ncolitems;
+    for (int k = 0; k < 4; ++k) L.part[k] = d_part.as<double>() + size_t(k) * nitem * kGapTerms;
+    L.cb = cb;
+    L.cg = cg;
+    L.lds_rows = lr;
+    L.lds_cols = lc;
+  }
+
+  int dtk() const {
+    return is_dbl(dtype) ? (is_cplx(dtype) ? DT_C128 : DT_F64) : (is_cplx(dtype) ? DT_C64 : DT_F32);
+  }
+
+  void reset_ctl() {
+    const int ns = (int)sv.size();
+    for (int i = 0; i < ns; ++i) {
+      int* c = h_ctl + size_t(i) * CTL_WORDS;
+      std::fill(c, c + CTL_WORDS, 0);
+      c[CTL_ACTIVE] = 1;
+      c[CTL_ITER] = 0;
+      c[CTL_RUPD] = sv[i]->rupd;
+      c[CTL_DONE_AT] = -1;
+    }
+    HIPC(hipMemcpyAsync(d_ctl.p, h_ctl, size_t(ns) * CTL_WORDS * sizeof(int), hipMemcpyHostToDevice, stream));
+    std::fill(h_stats, h_stats + size_t(ns) * ST_WORDS, 0.0);
+    HIPC(hipMemcpyAsync(d_stats.p, h_stats, size_t(ns) * ST_WORDS * sizeof(double), hipMemcpyHostToDevice, stream));
+  }
+
+  // Enqueues `len` iterations; the last one evaluates the gap when asked.
+  void enqueue(int len, bool gap) {
+    const int dt = dtk();
+    for (int i = 0; i < len; ++i) {
+      const bool g = gap && i == len - 1;
+      run_iteration(L, kind, dt, fp16, g);
+      if (g) run_gap_reduce(L, kind, dt);
+    }
+    const int ns = (int)sv.size();
+    HIPC(hipMemcpyAsync(h_ctl, d_ctl.p, size_t(ns) * CTL_WORDS * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPC(hipMemcpyAsync(h_stats, d_stats.p, size_t(ns) * ST_WORDS * sizeof(double), hipMemcpyDeviceToHost, stream));
+  }
+
+  void launch_chunk(int len, bool gap, bool use_graph) {
+    if (!use_graph) {
+      enqueue(len, gap);
+      HIPC(hipGetLastError());
+      return;
+    }
+    for (auto& g : graphs)
+      if (g.len == len && g.gap == gap) {
+        HIPC(hipGraphLaunch(g.exec, stream));
+        return;
+      }
+    hipGraph_t graph;
+    HIPC(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    enqueue(len, gap);
+    HIPC(hipStreamEndCapture(stream, &graph));
+    hipGraphExec_t exec;
+    HIPC(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    graphs.push_back({len, gap, exec});
+    HIPC(hipGraphLaunch(exec, stream));
+  }
+
+  void run(int verbose) {
+    const int ns = (int)sv.size();
+    const unsigned gap_every = sv[0]->prm.gap_every;
+    const bool gap = gap_every > 0;
+    std::vector<unsigned> limit(ns);
+    unsigned total = 0;
+    for (int i = 0; i < ns; ++i) {
+      limit[i] = sv[i]->prm.max_iter + 1;  // the reference runs k = 0..max_iter
+      total = std::max(total, limit[i]);
+    }
+    const unsigned chunk = gap ? gap_every : std::min<unsigned>(total, 50);
+    const bool use_graph = sv[0]->prm.use_graph != 0;
+    reset_ctl();
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    HIPC(hipEventRecord(e0, stream));
+    unsigned k = 0;
+    while (k < total) {
+      // chunk boundary: next multiple of the chunk, or the next slice limit
+      unsigned b = std::min(total, (k / chunk + 1) * chunk);
+      for (int i = 0; i < ns; ++i)
+        if (limit[i] > k) b = std::min(b, limit[i]);
+      const int len = int(b - k);
+      const bool g = gap && (b % gap_every) == 0;
+      launch_chunk(len, g, use_graph);
+      HIPC(hipStreamSynchronize(stream));
+      k = b;
+      // a slice stops at its own iteration limit (host-driven, so that no
+      // kernel ever reads a control word that another block of the same
+      // launch writes)
+      bool changed = false, any = false;
+      for (int i = 0; i < ns; ++i) {
+        int* c = h_ctl + size_t(i) * CTL_WORDS;
+        if (c[CTL_ACTIVE] && limit[i] <= k) {
+          c[CTL_ACTIVE] = 0;
+          changed = true;
+        }
+        any = any || c[CTL_ACTIVE];
+      }
+      if (verbose) {
+        for (int i = 0; i < std::min(ns, 4); ++i) {
+          const double* st = h_stats + size_t(i) * ST_WORDS;
+          std::printf("[pdt] slice %d iter=%d P=%.9g D=%.9g rel_gap=%.3e\n", i, h_ctl[i * CTL_WORDS + CTL_ITER],
+                      st[ST_P], st[ST_D], st[ST_REL]);
+        }
+        std::fflush(stdout);
+      }
+      if (!any) break;
+      if (changed)
+        HIPC(hipMemcpyAsync(d_ctl.p, h_ctl, size_t(ns) * CTL_WORDS * sizeof(int), hipMemcpyHostToDevice, stream));
+    }
+    HIPC(hipEventRecord(e1, stream));
+    HIPC(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (int i = 0; i < ns; ++i) {
+      pdt_solver* s = sv[i];
+      const int* c = h_ctl + size_t(i) * CTL_WORDS;
+      const double* st = h_stats + size_t(i) * ST_WORDS;
+      s->stats.iters = unsigned(c[CTL_ITER]);
+      s->stats.n_gap = unsigned(c[CTL_NGAP]);
+      s->stats.primal = st[ST_P];
+      s->stats.dual = st[ST_D];
+      s->stats.gap = st[ST_GAP];
+      s->stats.rel_gap = st[ST_REL];
+      s->stats.ms = ms;
+      s->stats.converged = c[CTL_DONE_AT] >= 0;
+      s->rupd = c[CTL_RUPD];
+    }
+  }
+};
+
+void check_solvable(pdt_solver* s) {
+  if (s->is_op() && !s->op) fail(PDT_ERR_STATE, "No operator is set!");  // tgvmrioptimizer.cu:96-102
+  if (!s->f) fail(PDT_ERR_STATE, "No noisy data is set!");              // :104-110
+  if (s->is_op() && !is_cplx(s->dtype)) fail(PDT_ERR_UNSUPPORTED, "operator solvers need complex images");
+}
+
+void prepare_solve(pdt_solver* s, int verbose) {
+  check_solvable(s);
+  if (!s->stream) HIPC(hipStreamCreate(&s->stream));
+  if (s->is_op()) {
+    if (s->op->dtype != s->dtype) fail(PDT_ERR_ARG, "operator and solver precision differ");
+    prepare_op_data(s);
+    if (!s->have_u) {
+      // Initialization with A^T(f) (tgvmrioptimizer.cu:112-118)
+      if (verbose) std::printf("Initialization with A^T(f)\n");
+      const size_t n = size_t(s->op->W) * s->op->H;
+      DevBuf u0(n * s->pe());
+      s->op->apply(false, s->f->ptr, u0.p, s->stream);
+      HIPC(hipStreamSynchronize(s->stream));
+      set_input0(s, u0.p, nullptr, s->op->W, s->op->H);
+    } else {
+      const int64_t want[2] = {s->op->W, s->op->H};
+      size_check(s->u->size, 2, want, 2);
+    }
+  } else {
+    if (s->f->ndim != 2) fail(PDT_ERR_SIZE, "denoising data must be 2D");
+    if (!s->have_u) {
+      // Initialization with f (tvoptimizer.cu:87-92)
+      if (verbose) std::printf("Initialization with f\n");
+      set_input0(s, s->f->ptr, nullptr, int(s->f->size[0]), int(s->f->size[1]));
+    } else {
+      size_check(s->u->size, 2, s->f->size, 2);
+    }
+  }
+  if (s->is_tgv() && s->kind == PDT_SOLVER_TGV && !s->tmp.p) s->tmp.alloc(2 * size_t(s->W) * s->H * s->pe());
+}
+
+void solve_many(std::vector<pdt_solver*>& sv, int verbose) {
+  for (auto* s : sv) prepare_solve(s, verbose);
+  for (size_t i = 1; i < sv.size(); ++i) {
+    if (sv[i]->kind != sv[0]->kind || sv[i]->dtype != sv[0]->dtype)
+      fail(PDT_ERR_ARG, "batched solvers must share kind and dtype");
+    if (sv[i]->prm.gap_every != sv[0]->prm.gap_every)
+      fail(PDT_ERR_ARG, "batched solvers must share gap_every");
+  }
+  Engine e;
+  e.sv = sv;
+  e.build();
+  e.run(verbose);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* pdt_last_error(void) { return g_err.c_str(); }
+int pdt_version(void) { return 100; }
+
+int pdt_set_device(int device) {
+  return guard([&] { HIPC(hipSetDevice(device)); });
+}
+
+int pdt_array_create(int dtype, int ndim, const int64_t* size, pdt_array** out) {
+  return guard([&] {
+    if (!out || !size) fail(PDT_ERR_ARG, "null argument");
+    *out = new_array(dtype, ndim, size);
+  });
+}
+
+int pdt_array_wrap(int dtype, int ndim, const int64_t* size, void* device_ptr, pdt_array** out) {
+  return guard([&] {
+    if (!out || !size) fail(PDT_ERR_ARG, "null argument");
+    if (ndim < 1 || ndim > 3) fail(PDT_ERR_ARG, "ndim must be 1..3");
+    auto a = std::make_unique<pdt_array>();
+    a->dtype = dtype;
+    a->ndim = ndim;
+    for (int i = 0; i < ndim; ++i) a->size[i] = size[i];
+    (void)esize(dtype);
+    a->ptr = device_ptr;
+    a->owned = false;
+    *out = a.release();
+  });
+}
+
+int pdt_array_upload(pdt_array* a, const void* host, size_t bytes) {
+  return guard([&] {
+    if (!a || !host) fail(PDT_ERR_ARG, "null argument");
+    if (bytes != a->bytes()) fail(PDT_ERR_SIZE, "upload size does not match the array");
+    if (bytes) HIPC(hipMemcpy(a->ptr, host, bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int pdt_array_download(const pdt_array* a, void* host, size_t bytes) {
+  return guard([&] {
+    if (!a || !host) fail(PDT_ERR_ARG, "null argument");
+    if (bytes != a->bytes()) fail(PDT_ERR_SIZE, "download size does not match the array");
+    if (bytes) HIPC(hipMemcpy(host, a->ptr, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int pdt_array_info(const pdt_array* a, int* dtype, int* ndim, int64_t* size3) {
+  return guard([&] {
+    if (!a) fail(PDT_ERR_ARG, "null array");
+    if (dtype) *dtype = a->dtype;
+    if (ndim) *ndim = a->ndim;
+    if (size3)
+      for (int i = 0; i < 3; ++i) size3[i] = a->size[i];
+  });
+}
+
+void* pdt_array_device_ptr(pdt_array* a) { return a ? a->ptr : nullptr; }
+
+int pdt_array_destroy(pdt_array* a) {
+  return guard([&] { delete a; });
+}
+
+// ---- operators ----
+int pdt_op_create(int kind, int dtype, pdt_op** out) {
+  return guard([&] {
+    if (!out) fail(PDT_ERR_ARG, "null argument");
+    if (kind < PDT_OP_CARTESIAN || kind > PDT_OP_SAMPLING) fail(PDT_ERR_ARG, "unknown operator kind");
+    if (dtype != PDT_C64 && dtype != PDT_C128) fail(PDT_ERR_ARG, "operators are complex (PDT_C64 / PDT_C128)");
+    auto op = std::make_unique<pdt_op>();
+    op->kind = kind;
+    op->dtype = dtype;
+    *out = op.release();
+  });
+}
+
+int pdt_op_add_constant(pdt_op* op, const void* host, int dtype, int ndim, const int64_t* size) {
+  return guard([&] {
+    if (!op || !host || !size) fail(PDT_ERR_ARG, "null argument");
+    // OperatorBase::addConstant (operatorbase.h:238-257)
+    if ((int)op->consts.size() >= op->required())
+      fail(PDT_ERR_CONST, op->prefix() + "Too many InputTypeConstants. Could not add new constant.'");
+    HostConst c;
+    c.dtype = dtype;
+    c.ndim = ndim;
+    for (int i = 0; i < 3; ++i) c.size[i] = i < ndim ? size[i] : 1;
+    c.dev.reset(new_array(dtype, ndim, size));
+    const size_t b = c.dev->bytes();
+    c.host.assign(static_cast<const char*>(host), static_cast<const char*>(host) + b);
+    if (b) HIPC(hipMemcpy(c.dev->ptr, host, b, hipMemcpyHostToDevice));
+    op->consts.push_back(std::move(c));
+    op->prepared = false;
+  });
+}
+
+int pdt_op_num_constants(const pdt_op* op, int* have, int* required) {
+  return guard([&] {
+    if (!op) fail(PDT_ERR_ARG, "null operator");
+    if (have) *have = (int)op->consts.size();
+    if (required) *required = op->required();
+  });
+}
+
+namespace {
+// sizeCheck of the three operators (mricartesianoperator.cu:173-180,310-318,
+// mrisamplingoperator.cu:39-44), in = image array, out = k-space array.
+void op_size_check(pdt_op* op, const pdt_array* in, const pdt_array* out) {
+  op->check_count();
+  const HostConst& m = op->mask();
+  if (op->kind == PDT_OP_SAMPLING) {
+    size_check(out->size, out->ndim, m.size, 2);
+    size_check(in->size, in->ndim, m.size, 2);
+    return;
+  }
+  const HostConst* s = op->sens();
+  size_check(out->size, out->ndim, s->size, 3);
+  if (op->kind == PDT_OP_CARTESIAN) {
+    size_check(in->size, in->ndim, m.size, 2);
+    const int64_t o2[2] = {out->size[0], out->size[1]};
+    size_check(in->size, in->ndim, o2, 2);
+  } else {
+    int64_t sz[2] = {in->size[0], in->size[1] * 2};
+    if (in->ndim != 2) fail(PDT_ERR_SIZE, "image must be 2D");
+    size_check(sz, 2, m.size, 2);
+  }
+  if (in->dtype != op->dtype || out->dtype != op->dtype) fail(PDT_ERR_ARG, "array dtype does not match the operator");
+}
+}  // namespace
+
+int pdt_op_forward(pdt_op* op, const pdt_array* src, pdt_array* dst) {
+  return guard([&] {
+    if (!op || !src || !dst) fail(PDT_ERR_ARG, "null argument");
+    op_size_check(op, src, dst);
+    op->prepare();
+    op->apply(true, src->ptr, dst->ptr, nullptr);
+    HIPC(hipStreamSynchronize(nullptr));
+  });
+}
+
+int pdt_op_adjoint(pdt_op* op, const pdt_array* src, pdt_array* dst) {
+  return guard([&] {
+    if (!op || !src || !dst) fail(PDT_ERR_ARG, "null argument");
+    op_size_check(op, dst, src);
+    op->prepare();
+    op->apply(false, src->ptr, dst->ptr, nullptr);
+    HIPC(hipStreamSynchronize(nullptr));
+  });
+}
+
+int pdt_op_input_size(pdt_op* op, int out_ndim, const int64_t* out_size, int* in_ndim, int64_t* in_size) {
+  return guard([&] {
+    if (!op || !out_size || !in_ndim || !in_size) fail(PDT_ERR_ARG, "null argument");
+    (void)out_ndim;
+    *in_ndim = 2;
+    in_size[0] = out_size[0];
+    in_size[1] = op->kind == PDT_OP_CARTESIAN_REMOVE_ROOS ? out_size[1] / 2 : out_size[1];
+  });
+}
+
+int pdt_op_output_size(pdt_op* op, int in_ndim, const int64_t* in_size, int* out_ndim, int64_t* out_size) {
+  return guard([&] {
+    if (!op || !out_ndim || !out_size) fail(PDT_ERR_ARG, "null argument");
+    (void)in_ndim;
+    (void)in_size;
+    if (op->kind == PDT_OP_SAMPLING) {
+      const HostConst& m = op->get_const(0, real_of(op->dtype), 2);
+      *out_ndim = 2;
+      out_size[0] = m.size[0];
+      out_size[1] = m.size[1];
+    } else {
+      const HostConst& s = op->get_const(0, cplx_of(op->dtype), 3);
+      *out_ndim = 3;
+      for (int i = 0; i < 3; ++i) out_size[i] = s.size[i];
+    }
+  });
+}
+
+int pdt_op_adjointness(pdt_op* op, double* lre, double* lim, double* rre, double* rim) {
+  return guard([&] {
+    if (!op) fail(PDT_ERR_ARG, "null operator");
+    op->prepare();
+    // iu::random::fillRandomFloatingNumbers (iurandom.h:120-136): U[0,1)
+    // pairs from std::default_random_engine seeded with a global seed that
+    // advances by the element count after every fill.
+    static unsigned seed = 42;
+    const bool dbl = is_dbl(op->dtype);
+    const size_t nu = size_t(op->W) * op->H, np = size_t(op->W) * op->Hk * op->C;
+    auto fill = [&](size_t n) {
+      std::vector<double> v(2 * n);
+      std::default_random_engine gen(seed);
+      std::uniform_real_distribution<> dis(0, 1);
+      for (size_t i = 0; i < n; ++i) {
+        v[2 * i] = dis(gen);
+        v[2 * i + 1] = dis(gen);
+      }
+      seed += unsigned(n);
+      return v;
+    };
+    std::vector<double> hu = fill(nu), hp = fill(np);
+    const size_t ce = esize(op->dtype);
+    DevBuf du(nu * ce), dp(np * ce), dAu(np * ce), dAtp(nu * ce);
+    auto up = [&](DevBuf& d, const std::vector<double>& h) {
+      if (dbl) {
+        HIPC(hipMemcpy(d.p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+      } else {
+        std::vector<float> f(h.begin(), h.end());
+        HIPC(hipMemcpy(d.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+      }
+    };
+    auto down = [&](DevBuf& d, size_t n) {
+      std::vector<double> h(2 * n);
+      if (dbl) {
+        HIPC(hipMemcpy(h.data(), d.p, h.size() * 8, hipMemcpyDeviceToHost));
+      } else {
+        std::vector<float> f(2 * n);
+        HIPC(hipMemcpy(f.data(), d.p, f.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < f.size(); ++i) h[i] = f[i];
+      }
+      return h;
+    };
+    up(du, hu);
+    up(dp, hp);
+    op->apply(true, du.p, dAu.p, nullptr);
+    op->apply(false, dp.p, dAtp.p, nullptr);
+    HIPC(hipStreamSynchronize(nullptr));
+    std::vector<double> Au = down(dAu, np), Atp = down(dAtp, nu);
+    // <a, b> = sum conj(a) b
+    auto dot = [](const std::vector<double>& a, const std::vector<double>& b, double& re, double& im) {
+      re = im = 0;
+      for (size_t i = 0; i < a.size() / 2; ++i) {
+        re += a[2 * i] * b[2 * i] + a[2 * i + 1] * b[2 * i + 1];
+        im += a[2 * i] * b[2 * i + 1] - a[2 * i + 1] * b[2 * i];
+      }
+    };
+    double a, b, c, d;
+    dot(Au, hp, a, b);
+    dot(hu, Atp, c, d);
+    if (lre) *lre = a;
+    if (lim) *lim = b;
+    if (rre) *rre = c;
+    if (rim) *rim = d;
+  });
+}
+
+int pdt_op_name(const pdt_op* op, char* buf, size_t len) {
+  return guard([&] {
+    if (!op || !buf || !len) fail(PDT_ERR_ARG, "null argument");
+    std::ostringstream ss;
+    ss << op->name() << ": Added constants: (" << op->consts.size() << "/" << op->required() << ")";
+    std::snprintf(buf, len, "%s", ss.str().c_str());
+  });
+}
+
+int pdt_op_destroy(pdt_op* op) {
+  return guard([&] { op_release(op); });
+}
+
+// ---- solvers ----
+int pdt_solver_create(int kind, int dtype, pdt_solver** out) {
+  return guard([&] {
+    if (!out) fail(PDT_ERR_ARG, "null argument");
+    if (kind < PDT_SOLVER_TV || kind > PDT_SOLVER_TGV_OP) fail(PDT_ERR_ARG, "unknown solver kind");
+    (void)esize(dtype);
+    auto s = std::make_unique<pdt_solver>();
+    s->kind = kind;
+    s->dtype = dtype;
+    default_params(kind, &s->prm);
+    if (s->is_op() && !is_cplx(dtype)) fail(PDT_ERR_UNSUPPORTED, "operator solvers need complex images");
+    *out = s.release();
+  });
+}
+
+int pdt_params_default(int kind, pdt_params* p) {
+  return guard([&] {
+    if (!p) fail(PDT_ERR_ARG, "null argument");
+    default_params(kind, p);
+  });
+}
+
+int pdt_solver_get_params(const pdt_solver* s, pdt_params* p) {
+  return guard([&] {
+    if (!s || !p) fail(PDT_ERR_ARG, "null argument");
+    *p = s->prm;
+  });
+}
+
+int pdt_solver_set_params(pdt_solver* s, const pdt_params* p) {
+  return guard([&] {
+    if (!s || !p) fail(PDT_ERR_ARG, "null argument");
+    const bool fp16_changed = (p->dual_fp16 != 0) != (s->prm.dual_fp16 != 0);
+    s->prm = *p;
+    if (fp16_changed && s->have_u) fail(PDT_ERR_STATE, "dual_fp16 must be set before the first solve");
+  });
+}
+
+int pdt_solver_set_operator(pdt_solver* s, pdt_op* op) {
+  return guard([&] {
+    if (!s || !op) fail(PDT_ERR_ARG, "null argument");
+    if (!s->is_op()) fail(PDT_ERR_ARG, "this solver takes no operator");
+    op->refs++;
+    op_release(s->op);
+    s->op = op;
+    s->data_dirty = true;
+  });
+}
+
+int pdt_solver_set_noisy_data(pdt_solver* s, const pdt_array* f) {
+  return guard([&] {
+    if (!s || !f) fail(PDT_ERR_ARG, "null argument");
+    if (f->dtype != s->dtype) fail(PDT_ERR_ARG, "data dtype does not match the solver");
+    s->f.reset(new_array(f->dtype, f->ndim, f->size));
+    if (f->bytes()) HIPC(hipMemcpy(s->f->ptr, f->ptr, f->bytes(), hipMemcpyDeviceToDevice));
+    s->data_dirty = true;  // r = 0 (tgvmrioptimizer.cu:88-90)
+  });
+}
+
+int pdt_solver_set_input0(pdt_solver* s, const pdt_array* u0, const pdt_array* v0) {
+  return guard([&] {
+    if (!s || !u0) fail(PDT_ERR_ARG, "null argument");
+    if (u0->dtype != s->dtype) fail(PDT_ERR_ARG, "input dtype does not match the solver");
+    if (u0->ndim != 2) fail(PDT_ERR_SIZE, "input must be 2D");
+    if (v0) {
+      if (!s->is_tgv()) fail(PDT_ERR_ARG, "v0 is only taken by the TGV solvers");
+      const int64_t want[3] = {u0->size[0], u0->size[1], 2};
+      size_check(v0->size, v0->ndim, want, 3);
+      if (v0->dtype != s->dtype) fail(PDT_ERR_ARG, "v0 dtype does not match the solver");
+    }
+    set_input0(s, u0->ptr, v0 ? v0->ptr : nullptr, int(u0->size[0]), int(u0->size[1]));
+  });
+}
+
+int pdt_solver_solve(pdt_solver* s, int verbose) {
+  return guard([&] {
+    if (!s) fail(PDT_ERR_ARG, "null solver");
+    std::vector<pdt_solver*> v{s};
+    solve_many(v, verbose);
+  });
+}
+
+int pdt_solver_get_result(pdt_solver* s, pdt_array** u) {
+  return guard([&] {
+    if (!s || !u) fail(PDT_ERR_ARG, "null argument");
+    if (!s->have_u) fail(PDT_ERR_STATE, "There is no result yet!");
+    *u = s->u.get();
+  });
+}
+
+int pdt_solver_get_v(pdt_solver* s, pdt_array** v) {
+  return guard([&] {
+    if (!s || !v) fail(PDT_ERR_ARG, "null argument");
+    if (!s->have_u) fail(PDT_ERR_STATE, "There is no result yet!");
+    if (!s->is_tgv()) fail(PDT_ERR_ARG, "this solver has no v");
+    *v = s->v.get();
+  });
+}
+
+int pdt_solver_get_stats(const pdt_solver* s, pdt_stats* st) {
+  return guard([&] {
+    if (!s || !st) fail(PDT_ERR_ARG, "null argument");
+    *st = s->stats;
+  });
+}
+
+int pdt_solver_destroy(pdt_solver* s) {
+  return guard([&] { delete s; });
+}
+
+int pdt_batch_solve(pdt_solver** solvers, int count, int verbose) {
+  return guard([&] {
+    if (!solvers || count <= 0) fail(PDT_ERR_ARG, "empty batch");
+    std::vector<pdt_solver*> v(solvers, solvers + count);
+    for (auto* s : v)
+      if (!s) fail(PDT_ERR_ARG, "null solver in batch");
+    solve_many(v, verbose);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// pdt_desc.hpp -- descriptors shared by the host engine and the HIP kernels.
+//
+// One "slice" is one reconstruction problem (one image, its data, its duals).
+// A single solve is a batch of one slice; a batched solve packs many slices of
+// uneven size and coil count into the same launches.  Kernels receive an array
+// of slice descriptors plus a work list of (slice, index) items, so every
+// launch is a persistent work-list over the whole batch.
+#pragma once
+
+#include <cstdint>
+
+namespace pdt {
+
+constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+constexpr int kMaxStages = 14;    // FFT stages per transform
+constexpr int kGapTerms = 8;      // partial-sum slots per work item
+constexpr int kMaxRowLen = 1024;  // W limit of the MRI row kernels (4 px / thread)
+
+// Solver kinds (match PDT_SOLVER_* of pdt_capi.h).
+enum Kind { K_TV = 0, K_TGV = 1, K_TGV_MRI = 2, K_TV_OP = 3, K_TGV_OP = 4 };
+
+// Mixed-radix Stockham plan of one transform length n.
+struct FftPlanD {
+  int n;
+  int ns;
+  int r[kMaxStages];
+  const void* tw;  // Cx<R>[n]: exp(-2 pi i m / n)
+};
+
+// Control words of a slice (device memory).
+enum Ctl { CTL_ACTIVE = 0, CTL_ITER = 1, CTL_RUPD = 2, CTL_DONE_AT = 3, CTL_NGAP = 4, CTL_WORDS = 8 };
+// Statistics doubles of a slice (device memory), written by the gap reduction.
+enum Stat { ST_P = 0, ST_D = 1, ST_GAP = 2, ST_REL = 3, ST_C0 = 4, ST_C1 = 5, ST_C2 = 6, ST_WORDS = 8 };
+
+struct SliceDev {
+  // geometry
+  int W, H;       // image {W (x, contiguous), H (y)}
+  int Hk, y0, C;  // k-space FE length, FE offset of the image window, coils
+  int ncols;      // listed (sampled) PE columns
+  int ngroups;    // column groups of the column pass (per coil)
+  int kind;
+  // scalar parameters
+  double tau, sigma, theta, lambda;
+  double alpha0, alpha1;          // alpha at k = 0
+  double alpha0_end, alpha1_end;  // alpha * reduction (TGV-MRI schedule)
+  double kmax;                    // max_iter as a real, for the schedule
+  int schedule;                   // 1: TGV-MRI alpha continuation
+  int dual_fp16;
+  double radius_u, radius_v;      // partial-gap ball radii
+  double F0;                      // ||f||^2 on columns without samples
+  double gap_tol;
+  // image fields, planes of W*H pixels
+  void* u;
+  void* ub;
+  void* v;   // 2 planes (x, y)
+  void* vb;  // 2 planes
+  void* p;   // 2 planes
+  void* q;   // 3 planes (xx, yy, xy)
+  const void* f;  // denoising data
+  void* tmp;      // 2 planes: TGV dual-restoration scratch
+  // k-space (operator solvers)
+  void* fk;         // compact data [c][ks][kfe]
+  void* rk;         // compact dual r [c][ks][kfe]
+  const void* S;    // coil sensitivities {W, Hk, C}; null for the sampling op
+  const void* Mc;   // compact mask [ks][kfe] (real)
+  const int* cols;    // ks -> kx
+  const int* colmap;  // kx -> ks or -1
+  void* T1;         // row-pass output [c][y][ks]
+  void* T2;         // column-pass output [c][y][ks]
+  const void* op_in;  // operator mode input
+  void* op_out;       // operator mode output
+  FftPlanD planW, planH;
+  // control and results
+  int* ctl;
+  double* stats;
+  // partial-sum ranges of this slice in the row and column work lists
+  int row_beg, row_end, col_beg, col_end;
+};
+
+}  // namespace pdt

@@ -1,0 +1,47 @@
+// pdt_launch.hpp -- host entry points into the HIP kernels (pdt_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pdt_desc.hpp"
+
+namespace pdt {
+
+// Element types (match PDT_F32 / PDT_C64 / PDT_F64 / PDT_C128).
+enum DType { DT_F32 = 0, DT_C64 = 1, DT_F64 = 2, DT_C128 = 3 };
+
+struct Launch {
+  hipStream_t stream = nullptr;
+  const SliceDev* sl = nullptr;  // device array of slices
+  int nsl = 0;
+  const int2* rows = nullptr;    // (slice, first row) items
+  int nrows = 0;
+  int rpi = 1;                   // rows per item (denoising kernels)
+  const int2* cols = nullptr;    // (slice, coil * ngroups + group) items
+  int ncolitems = 0;
+  double* part[4] = {nullptr, nullptr, nullptr, nullptr};
+  int cb = 1;                    // coils per LDS chunk in the row kernels
+  int cg = 1;                    // columns per group in the column kernel
+  size_t lds_rows = 0;           // dynamic LDS bytes of the row kernels
+  size_t lds_cols = 0;           // dynamic LDS bytes of the column kernel
+};
+
+// One full iteration of the solver kind on every active slice.
+void run_iteration(const Launch& L, int kind, int dtype, bool fp16, bool gap);
+// Gap evaluation from the partial sums of the last iteration (one block/slice).
+void run_gap_reduce(const Launch& L, int kind, int dtype);
+// Operator application in operator mode (op_in -> op_out of every slice).
+void run_op_forward(const Launch& L, int dtype);
+void run_op_adjoint(const Launch& L, int dtype);
+// Gathers full-layout data {W, Hk, C} into the compact [c][ks][kfe] layout,
+// zeroes r, and writes per-block partial sums of |f|^2 over columns without
+// samples into f0_part (nblocks entries, returned).
+int run_compact_data(const Launch& L, int dtype, const void* f_full, double* f0_part);
+// Fills n elements of the given size (bytes per element 4, 8 or 16) with 0.
+void run_zero(hipStream_t st, void* ptr, size_t bytes);
+
+// LDS bytes needed by the row / column kernels.
+size_t lds_rows_bytes(int dtype, int cb, int W);
+size_t lds_cols_bytes(int dtype, int cg, int Hk);
+
+}  // namespace pdt
