@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark: time-to-gap of the Cartesian TGV-MRI solver on one MI355X.
+
+Workload (one "step" = one cold-start solve to relative primal-dual gap 1e-4):
+  320 x 320 complex64 image (ellipse phantom, quadratic ramp, polynomial
+  phase), 15 coils (Gaussian bumps on a circle, smooth phase, sum |S|^2 = 1),
+  read-out oversampled k-space {320 PE, 640 FE}, random 4x PE-line mask with
+  24 fully sampled centre lines, complex noise at 30 dB SNR,
+  MriCartesianRemoveROOSOperator, TgvMriOptimizer with alpha1 = 1e-5 ||f||,
+  alpha0 = 2 alpha1, reduction 1, gap evaluated every 50 iterations.
+  Cold start u0 = A^* f (v = p = q = r = 0), as the reference's solve().
+
+Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "primal-dual-toolbox_amd"))
+
+HBM_PEAK_GBS = 8000.0
+GAP_TOL = 1e-4
+GAP_EVERY = 50
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(seed: int = 0):
+    from primaldualtoolbox import synthetic as syn
+    case = syn.mri_case(seed=seed, n=320, coils=15, accel=4.0, center=24, snr_db=30.0, roos=True)
+    a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
+    return case, a1
+
+
+def build_solver(case, a1, max_iter=200000):
+    import primaldualtoolbox as pdt
+    op = pdt.mri.MriCartesianRemoveROOSOperator()
+    op.setCoilSens(case["sens"])
+    op.setMask(case["mask"])
+    opt = pdt.mri.TgvMriOptimizer_2c3c()
+    opt.setOperator(op)
+    opt.setNoisyData(case["kspace"])
+    prm = opt.getParameters()
+    prm.alpha1, prm.alpha0, prm.reduction = a1, 2 * a1, 1.0
+    prm.max_iter = max_iter
+    prm.gap_tol, prm.gap_every = GAP_TOL, GAP_EVERY
+    return op, opt
+
+
+def cold_start(opt, u0_dev):
+    """setInput0(A^* f) from a device-resident copy (v, p, q zeroed)."""
+    from primaldualtoolbox import _capi as C
+    C.check(C.lib().pdt_solver_set_input0(opt._h, u0_dev.h, None))
+
+
+def cpu_baseline(case, a1, gpu_iters: int, sample_iters: int, threads: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.set_threads(threads)
+    op = O.Op(O.OP_CARTESIAN_ROOS, case["sens"], case["mask"], prec=32)
+    _, _, st, _ = O.solve(O.TGV_MRI, case["kspace"], op=op, alpha1=a1, alpha0=2 * a1,
+                          max_iter=sample_iters - 1, prec=32)
+    per_iter = st["seconds"] / sample_iters
+    return {
+        "value": per_iter * gpu_iters,
+        "unit": "s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{sample_iters} TGV-MRI iterations of the same 320x320x15 workload in the FP32 "
+                   f"OpenMP oracle ({per_iter * 1e3:.1f} ms/iteration), scaled to the GPU's "
+                   f"{gpu_iters} iterations to rel. gap {GAP_TOL:g}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(slices shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from primaldualtoolbox import _capi as C
+
+    log("[bench] generating workload (seed 0)")
+    case, a1 = make_workload()
+    op, opt = build_solver(case, a1)
+    # u0 = A^* f on the device, kept resident for every cold start
+    f_dev = C.DeviceArray.from_numpy(case["kspace"], np.complex64)
+    H, W = case["image"].shape
+    u0_dev = C.DeviceArray.empty(C.PDT_C64, [W, H])
+    C.check(C.lib().pdt_op_adjoint(op._h, f_dev.h, u0_dev.h))
+
+    import primaldualtoolbox as pdt
+    tune = pdt.mri.autotune([opt], reps=20)
+    log(f"[bench] autotuned tiles: {tune}")
+
+    iters = []
+    gaps = []
+    loop_ms = []
+
+    def one_step():
+        cold_start(opt, u0_dev)
+        opt.solve()
+        st = opt.getStats()
+        iters.append(int(st["iters"]))
+        gaps.append(float(st["rel_gap"]))
+        loop_ms.append(float(st["ms"]))
+        if not st["converged"]:
+            raise RuntimeError(f"solve did not reach rel. gap {GAP_TOL}: {st}")
+
+    # r must restart at 0 for every cold solve: setNoisyData re-zeroes it.
+    for w in range(args.warmup):
+        opt.setNoisyData(case["kspace"])
+        one_step()
+        log(f"[bench] warmup {w}: {iters[-1]} iterations, rel gap {gaps[-1]:.3e}, loop {loop_ms[-1]:.1f} ms")
+    iters.clear(), gaps.clear(), loop_ms.clear()
+    step_s = []
+    for k in range(args.steps):
+        opt.setNoisyData(case["kspace"])  # untimed: host->device upload of f, r = 0
+        t0 = time.perf_counter()
+        one_step()
+        step_s.append(time.perf_counter() - t0)
+        log(f"[bench] step {k}: {step_s[-1]:.4f} s, {iters[-1]} iterations, rel gap {gaps[-1]:.3e}")
+    value = float(np.mean(step_s))
+
+    # per-kernel durations (hipEvents on the solver stream) and algorithmic bytes
+    prof = opt.profileKernels(args.profile_reps)
+    per_iter_ms = sum(p["ms"] for p in prof)
+    dom = max(prof, key=lambda p: p["ms"])
+    achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_cfg3_tgv_mri.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get(dom["name"])
+    kernels = {p["name"]: {"ms": round(p["ms"], 5), "alg_MB": round(p["bytes"] / 1e6, 3),
+                           "GBs": round(p["bytes"] / (p["ms"] * 1e-3) / 1e9, 1)} for p in prof}
+    log(f"[bench] kernels: {kernels}")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline: {args.cpu_iters} iterations on {args.cpu_threads} threads")
+        cpu = cpu_baseline(case, a1, int(np.median(iters)), args.cpu_iters, args.cpu_threads)
+
+    line = {
+        "metric": "time-to-gap 1e-4 (s), 15-coil 320^2 Cartesian TGV, cold start",
+        "value": value,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": value * 1e3,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "Cartesian TGV-MRI 320x320 complex64, 15 coils, ROOS grid 320 PE x 640 FE, random 4x "
+                        "PE-line mask (24 centre lines), 30 dB SNR, alpha1=1e-5||f||, alpha0=2 alpha1, "
+                        "rel. partial gap 1e-4 checked every 50 iterations, cold start u0=A^*f",
+            "iterations_to_gap": int(np.median(iters)),
+            "final_rel_gap": float(np.median(gaps)),
+            "device_loop_ms": float(np.median(loop_ms)),
+            "us_per_iteration": float(np.median(loop_ms)) * 1e3 / max(1, int(np.median(iters))),
+            "kernel_ms_per_iteration": per_iter_ms,
+            "kernels": kernels,
+            "tiles": tune,
+        },
+        "roofline": {
+            "kernel": dom["name"],
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -169,6 +169,29 @@ PDT_API int pdt_solver_get_v(pdt_solver* s, pdt_array** v);
 PDT_API int pdt_solver_get_stats(const pdt_solver* s, pdt_stats* st);
 PDT_API int pdt_solver_destroy(pdt_solver* s);
 
+/* NEW: per-kernel timing of a configured solver.  Each fused kernel of one
+ * iteration is launched `reps` times back to back on the solver's stream,
+ * bracketed by hipEvents; reported per kernel: the average launch duration
+ * and the algorithmic HBM bytes of one launch (every distinct element the
+ * launch reads or writes, once, at storage precision).  Starts from the
+ * solver's current state and leaves it advanced by arbitrary steps: call it
+ * after the solves whose results are used. */
+typedef struct pdt_kernel_prof {
+  char name[32];
+  double ms;
+  double bytes;
+} pdt_kernel_prof;
+PDT_API int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, int cap, int* n);
+
+/* NEW: autotuner.  Times every candidate tile shape (coils per LDS chunk and
+ * columns per group of the MRI passes, rows per work item of the denoising
+ * passes) on the configured solvers (one batch) with hipEvents, keeps the
+ * fastest in the process-wide tile cache (persisted to the file named by the
+ * environment variable PDT_TUNE_CACHE, if set) and returns it.  Advances the
+ * solver state: call before the solves and reset with set_input0. */
+PDT_API int pdt_autotune(pdt_solver** solvers, int count, int reps, int* cb, int* cg, int* rpi,
+                         double* ms_per_iteration);
+
 /* NEW: solve many configured solvers (same kind and dtype) together, with one
  * persistent work-list launch per half-iteration for the whole batch.  Each
  * solver keeps its own parameters, sizes, coils and stopping. */

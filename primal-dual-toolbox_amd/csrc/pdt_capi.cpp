@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -137,7 +138,16 @@ FftPlanD make_plan(int n, bool dbl) {
   for (int p : f)
     if (p > 64) fail(PDT_ERR_UNSUPPORTED, "FFT length has a prime factor above 64: " + std::to_string(n));
   pl.ns = (int)f.size();
-  for (size_t i = 0; i < f.size(); ++i) pl.r[i] = f[i];
+  int pp = 1;
+  for (size_t i = 0; i < f.size(); ++i) {
+    pl.r[i] = f[i];
+    pl.nr[i] = n / f[i];
+    pl.p[i] = pp;
+    pl.tws[i] = n / (pp * f[i]);
+    pl.rnr[i] = 1.0f / float(pl.nr[i]);
+    pl.rp[i] = 1.0f / float(pp);
+    pp *= f[i];
+  }
   int dev = 0;
   HIPC(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_tw_mu);
@@ -171,7 +181,21 @@ FftPlanD make_plan(int n, bool dbl) {
   return pl;
 }
 
-constexpr size_t kLdsBudget = 64 * 1024;
+// Largest tiles that fit the LDS of a workgroup: coils per row work item and
+// columns per column group of the MRI passes.
+int max_cb(int dt, int W, int nc, int C) {
+  int cb = std::max(C, 1);
+  while (cb > 1 && lds_rows_bytes(dt, cb, W, nc) > init_kernel_attrs()) --cb;
+  return cb;
+}
+int max_cg(int dt, int Hk, int nc) {
+  int cg = std::max(nc, 1);
+  while (cg > 1 && lds_cols_bytes(dt, cg, Hk) > init_kernel_attrs()) --cg;
+  return cg;
+}
+// Default tiles before any autotuning: 4 coils per row item, 2 columns per
+// column group.
+constexpr int kDefaultCb = 4, kDefaultCg = 2;
 
 }  // namespace
 
@@ -245,8 +269,8 @@ struct pdt_op {
   std::vector<int> h_cols;
   FftPlanD planW{}, planH{};
   // operator-mode scratch
-  DevBuf T1, T2, ctl, slice, rows, colitems;
-  int nrows = 0, ncolitems = 0, cb = 1, cg = 1;
+  DevBuf T1, T2, wpart, ctl, slice, rows, rowgroups, colitems;
+  int nrows = 0, nrg = 0, ncolitems = 0, cb = 1, cg = 1;
   size_t lds_rows = 0, lds_cols = 0;
   hipStream_t stream = nullptr;
 
@@ -362,13 +386,11 @@ struct pdt_op {
   // LDS chunking shared by the operator mode and the solvers
   void choose_tiles(int& cb_out, int& cg_out, size_t& lr, size_t& lc) const {
     const int dt = dtype == PDT_C128 ? DT_C128 : DT_C64;
-    const size_t e = is_dbl(dtype) ? 16 : 8;
-    int cbm = int(kLdsBudget / (2 * size_t(W) * e));
-    cb_out = std::max(1, std::min(C, cbm));
-    int cgm = int(kLdsBudget / (2 * size_t(Hk) * e));
-    cg_out = std::max(1, std::min({4, std::max(ncols, 1), cgm}));
-    if (cgm < 1) fail(PDT_ERR_UNSUPPORTED, prefix() + "FE length too large for the LDS column pass");
-    lr = lds_rows_bytes(dt, cb_out, W);
+    if (lds_cols_bytes(dt, 1, Hk) > init_kernel_attrs() || lds_rows_bytes(dt, 1, W, ncols) > init_kernel_attrs())
+      fail(PDT_ERR_UNSUPPORTED, prefix() + "transform length too large for the LDS passes");
+    cb_out = std::min(kDefaultCb, max_cb(dt, W, ncols, C));
+    cg_out = std::min(kDefaultCg, max_cg(dt, Hk, ncols));
+    lr = lds_rows_bytes(dt, cb_out, W, ncols);
     lc = lds_cols_bytes(dt, cg_out, Hk);
   }
 
@@ -384,12 +406,19 @@ struct pdt_op {
       HIPC(hipMemcpy(ctl.p, h, sizeof(h), hipMemcpyHostToDevice));
     }
     choose_tiles(cb, cg, lds_rows, lds_cols);
+    const int ngr = (C + cb - 1) / cb;
+    wpart.ensure(size_t(ngr) * H * W * ce);
     if (!rows.p) {
-      std::vector<int2> r(H);
+      std::vector<int2> r(H), rg;
       for (int y = 0; y < H; ++y) r[y] = make_int2(0, y);
+      for (int y = 0; y < H; ++y)
+        for (int g = 0; g < ngr; ++g) rg.push_back(make_int2(0, y * ngr + g));
       rows.alloc(r.size() * sizeof(int2));
       HIPC(hipMemcpy(rows.p, r.data(), r.size() * sizeof(int2), hipMemcpyHostToDevice));
       nrows = H;
+      rowgroups.alloc(rg.size() * sizeof(int2));
+      HIPC(hipMemcpy(rowgroups.p, rg.data(), rg.size() * sizeof(int2), hipMemcpyHostToDevice));
+      nrg = (int)rg.size();
       const int ng = (ncols + cg - 1) / cg;
       std::vector<int2> c;
       for (int cc = 0; cc < C; ++cc)
@@ -423,8 +452,10 @@ struct pdt_op {
     ensure_opmode();
     SliceDev d = geometry();
     d.ngroups = (ncols + cg - 1) / cg;
+    d.ngr = (C + cb - 1) / cb;
     d.T1 = T1.p;
     d.T2 = T2.p;
+    d.wpart = wpart.p;
     d.op_in = src;
     d.op_out = dst;
     d.ctl = ctl.as<int>();
@@ -435,6 +466,8 @@ struct pdt_op {
     L.nsl = 1;
     L.rows = rows.as<int2>();
     L.nrows = nrows;
+    L.rowgroups = rowgroups.as<int2>();
+    L.nrg = nrg;
     L.cols = colitems.as<int2>();
     L.ncolitems = ncolitems;
     L.cb = cb;
@@ -479,7 +512,7 @@ struct pdt_solver {
   bool have_u = false;
   std::unique_ptr<pdt_array> u, v;  // result handles (u: {W,H}, v: {W,H,2})
   DevBuf ub, vb, p, q, tmp;
-  DevBuf fk, rk, T1, T2;
+  DevBuf fk, rk, T1, T2, wpart;
   int Hk = 0, C = 0, ncols = 0;
   double F0 = 0, atf_norm = 0;
   int rupd = 0;  // r updates since set_noisy_data
@@ -628,12 +661,14 @@ void prepare_op_data(pdt_solver* s) {
 struct Engine {
   int kind = 0, dtype = 0;
   bool fp16 = false;
+  int force_cb = 0, force_cg = 0, force_rpi = 0;  // autotuner overrides (0: cache / default)
+  TuneKey key{};
   std::vector<pdt_solver*> sv;
   hipStream_t stream = nullptr;
-  DevBuf d_slices, d_rows, d_cols, d_part, d_ctl, d_stats;
+  DevBuf d_slices, d_rows, d_rg, d_cols, d_part, d_ctl, d_stats;
   int* h_ctl = nullptr;       // pinned mirrors
   double* h_stats = nullptr;
-  int nrows = 0, ncolitems = 0;
+  int nrows = 0, nrg = 0, ncolitems = 0;
   Launch L;
   std::vector<SliceDev> hs;
   struct GraphEntry {
@@ -671,28 +706,28 @@ struct Engine {
     }
     const int dt = is_dbl(dtype) ? (is_cplx(dtype) ? DT_C128 : DT_F64) : (is_cplx(dtype) ? DT_C64 : DT_F32);
     if (mri) {
-      const size_t e = is_dbl(dtype) ? 16 : 8;
+      init_kernel_attrs();
       int maxC = 1, maxNc = 1;
       for (auto* s : sv) {
         maxC = std::max(maxC, s->op->C);
         maxNc = std::max(maxNc, s->op->ncols);
       }
-      TuneKey key{kind, dt, maxW, sv[0]->H, maxHk, maxC, maxNc, ns};
+      key = TuneKey{kind, dt, maxW, sv[0]->H, maxHk, maxC, maxNc, ns};
       TuneVal tv = tune_lookup(key);
-      cb = tv.cb > 0 ? tv.cb : maxC;
-      cg = tv.cg > 0 ? tv.cg : 4;
-      cb = std::max(1, std::min({cb, maxC, int(kLdsBudget / (2 * size_t(maxW) * e))}));
-      cg = std::max(1, std::min({cg, maxNc, int(kLdsBudget / (2 * size_t(maxHk) * e))}));
-      lr = lds_rows_bytes(dt, cb, maxW);
+      cb = force_cb > 0 ? force_cb : tv.cb > 0 ? tv.cb : kDefaultCb;
+      cg = force_cg > 0 ? force_cg : tv.cg > 0 ? tv.cg : kDefaultCg;
+      cb = std::max(1, std::min(cb, max_cb(dt, maxW, maxNc, maxC)));
+      cg = std::max(1, std::min(cg, max_cg(dt, maxHk, maxNc)));
+      lr = lds_rows_bytes(dt, cb, maxW, maxNc);
       lc = lds_cols_bytes(dt, cg, maxHk);
     } else {
-      TuneKey key{kind, dt, maxW, sv[0]->H, 0, 0, 0, ns};
+      key = TuneKey{kind, dt, maxW, sv[0]->H, 0, 0, 0, ns};
       TuneVal tv = tune_lookup(key);
-      rpi = tv.rpi > 0 ? tv.rpi : 1;
+      rpi = force_rpi > 0 ? force_rpi : tv.rpi > 0 ? tv.rpi : 1;
     }
     // slices and work lists
     hs.assign(ns, SliceDev{});
-    std::vector<int2> rows, cols;
+    std::vector<int2> rows, rgs, cols;
     for (int i = 0; i < ns; ++i) {
       pdt_solver* s = sv[i];
       SliceDev d = mri ? s->op->geometry() : SliceDev{};
@@ -750,6 +785,9 @@ struct Engine {
         d.T1 = s->T1.p;
         d.T2 = s->T2.p;
         d.ngroups = (d.ncols + cg - 1) / cg;
+        d.ngr = (d.C + cb - 1) / cb;
+        s->wpart.ensure(size_t(d.ngr) * d.W * d.H * s->pe());
+        d.wpart = s->wpart.p;
       }
       d.ctl = d_ctl.as<int>() + size_t(i) * CTL_WORDS;
       d.stats = d_stats.as<double>() + size_t(i) * ST_WORDS;
@@ -757,6 +795,11 @@ struct Engine {
       const int step = mri ? 1 : rpi;
       for (int y = 0; y < s->H; y += step) rows.push_back(make_int2(i, y));
       d.row_end = (int)rows.size();
+      d.rg_beg = (int)rgs.size();
+      if (mri)
+        for (int y = 0; y < s->H; ++y)
+          for (int g = 0; g < d.ngr; ++g) rgs.push_back(make_int2(i, y * d.ngr + g));
+      d.rg_end = (int)rgs.size();
       d.col_beg = (int)cols.size();
       if (mri)
         for (int c = 0; c < d.C; ++c)
@@ -765,20 +808,25 @@ struct Engine {
       hs[i] = d;
     }
     nrows = (int)rows.size();
+    nrg = (int)rgs.size();
     ncolitems = (int)cols.size();
     d_slices.alloc(hs.size() * sizeof(SliceDev));
     HIPC(hipMemcpy(d_slices.p, hs.data(), hs.size() * sizeof(SliceDev), hipMemcpyHostToDevice));
     d_rows.alloc(std::max<size_t>(1, rows.size()) * sizeof(int2));
     HIPC(hipMemcpy(d_rows.p, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
+    d_rg.alloc(std::max<size_t>(1, rgs.size()) * sizeof(int2));
+    if (!rgs.empty()) HIPC(hipMemcpy(d_rg.p, rgs.data(), rgs.size() * sizeof(int2), hipMemcpyHostToDevice));
     d_cols.alloc(std::max<size_t>(1, cols.size()) * sizeof(int2));
     if (!cols.empty()) HIPC(hipMemcpy(d_cols.p, cols.data(), cols.size() * sizeof(int2), hipMemcpyHostToDevice));
-    const size_t nitem = std::max<size_t>(1, std::max(rows.size(), cols.size()));
+    const size_t nitem = std::max<size_t>(1, std::max({rows.size(), rgs.size(), cols.size()}));
     d_part.alloc(4 * nitem * kGapTerms * sizeof(double));
     L.stream = stream;
     L.sl = d_slices.as<SliceDev>();
     L.nsl = ns;
     L.rows = d_rows.as<int2>();
     L.nrows = nrows;
+    L.rowgroups = d_rg.as<int2>();
+    L.nrg = nrg;
     L.rpi = rpi;
     L.cols = d_cols.as<int2>();
     L.ncolitems = ncolitems;
@@ -787,6 +835,32 @@ struct Engine {
     L.cg = cg;
     L.lds_rows = lr;
     L.lds_cols = lc;
+  }
+
+  // Algorithmic HBM bytes of one launch of a stage, summed over the slices.
+  double stage_bytes(int stage) const {
+    double tot = 0;
+    for (size_t i = 0; i < sv.size(); ++i) {
+      const pdt_solver* s = sv[i];
+      const double N = double(s->W) * s->H, pe = double(s->pe()), pq = double(s->pq_e());
+      if (kind == PDT_SOLVER_TV) {
+        tot += stage == 0 ? (pe + 4 * pq) * N : (2 * pe + pe + 2 * pq + pe) * N;
+      } else if (kind == PDT_SOLVER_TGV) {
+        tot += stage == 0 ? (10 * pe + 5 * pq) * N : (3 * pe + 10 * pq) * N;
+      } else {
+        const SliceDev& d = hs[i];
+        const double C = d.C, nc = d.ncols, Hk = d.Hk, H = d.H;
+        const double S = s->op->kind == PDT_OP_SAMPLING ? 0.0 : C * N * pe;  // sensitivities on the image window
+        const double T = C * H * nc * pe;                                   // T1 or T2
+        const bool tgv = kind != PDT_SOLVER_TV_OP;
+        const double Wp = double(d.ngr) * N * pe;  // per-coil-group partial adjoints
+        if (stage == 0) tot += (tgv ? 13 * pe : 5 * pe) * N + S + T;
+        else if (stage == 1) tot += 2 * T + 3 * C * nc * Hk * pe + nc * Hk * (pe / 2);
+        else if (stage == 2) tot += T + S + Wp;
+        else tot += Wp + (tgv ? 14 * pe : 5 * pe) * N;
+      }
+    }
+    return tot;
   }
 
   int dtk() const {
@@ -1339,6 +1413,110 @@ int pdt_solver_get_stats(const pdt_solver* s, pdt_stats* st) {
 
 int pdt_solver_destroy(pdt_solver* s) {
   return guard([&] { delete s; });
+}
+
+int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, int cap, int* n) {
+  return guard([&] {
+    if (!s || !out || !n || reps <= 0) fail(PDT_ERR_ARG, "bad argument");
+    prepare_solve(s, 0);
+    Engine e;
+    e.sv = {s};
+    e.build();
+    e.reset_ctl();
+    const int ns = num_stages(e.kind);
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    const int dt = e.dtk();
+    *n = std::min(ns, cap);
+    for (int st = 0; st < *n; ++st) {
+      run_stage(e.L, e.kind, dt, e.fp16, false, st);  // warm the code object
+      HIPC(hipEventRecord(e0, e.stream));
+      for (int r = 0; r < reps; ++r) run_stage(e.L, e.kind, dt, e.fp16, false, st);
+      HIPC(hipEventRecord(e1, e.stream));
+      HIPC(hipEventSynchronize(e1));
+      HIPC(hipGetLastError());
+      float ms = 0;
+      HIPC(hipEventElapsedTime(&ms, e0, e1));
+      std::snprintf(out[st].name, sizeof(out[st].name), "%s", stage_name(e.kind, st));
+      out[st].ms = double(ms) / reps;
+      out[st].bytes = e.stage_bytes(st);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}
+
+namespace {
+// Device time of one iteration (sum of the stage kernels), each stage timed
+// over `reps` back-to-back launches.
+double time_iteration(Engine& e, int reps) {
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  const int dt = e.dtk();
+  double tot = 0;
+  for (int st = 0; st < num_stages(e.kind); ++st) {
+    run_stage(e.L, e.kind, dt, e.fp16, false, st);
+    HIPC(hipEventRecord(e0, e.stream));
+    for (int r = 0; r < reps; ++r) run_stage(e.L, e.kind, dt, e.fp16, false, st);
+    HIPC(hipEventRecord(e1, e.stream));
+    HIPC(hipEventSynchronize(e1));
+    HIPC(hipGetLastError());
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, e0, e1));
+    tot += double(ms) / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return tot;
+}
+}  // namespace
+
+int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* out_cg, int* out_rpi,
+                 double* out_ms) {
+  return guard([&] {
+    if (!solvers || count <= 0 || reps <= 0) fail(PDT_ERR_ARG, "bad argument");
+    std::vector<pdt_solver*> sv(solvers, solvers + count);
+    for (auto* s : sv) prepare_solve(s, 0);
+    const bool mri = sv[0]->is_op();
+    std::vector<std::array<int, 3>> cand;
+    if (mri) {
+      int maxC = 1;
+      for (auto* s : sv) maxC = std::max(maxC, s->op->C);
+      std::vector<int> cbs{1, 2, 3, 4, 5, 8, (maxC + 1) / 2, maxC};
+      std::sort(cbs.begin(), cbs.end());
+      cbs.erase(std::unique(cbs.begin(), cbs.end()), cbs.end());
+      for (int cb : cbs)
+        if (cb <= maxC)
+          for (int cg : {1, 2, 4}) cand.push_back({cb, cg, 0});
+    } else {
+      for (int rpi : {1, 2, 4, 8}) cand.push_back({0, 0, rpi});
+    }
+    double best = 1e300;
+    std::array<int, 3> bc{0, 0, 0};
+    TuneKey key{};
+    for (const auto& c : cand) {
+      Engine e;
+      e.sv = sv;
+      e.force_cb = c[0];
+      e.force_cg = c[1];
+      e.force_rpi = c[2];
+      e.build();
+      e.reset_ctl();
+      const double ms = time_iteration(e, reps);
+      key = e.key;
+      if (ms < best) {
+        best = ms;
+        bc = {e.L.cb, e.L.cg, e.L.rpi};
+      }
+    }
+    tune_store(key, TuneVal{mri ? bc[0] : 0, mri ? bc[1] : 0, mri ? 0 : bc[2]});
+    if (out_cb) *out_cb = bc[0];
+    if (out_cg) *out_cg = bc[1];
+    if (out_rpi) *out_rpi = bc[2];
+    if (out_ms) *out_ms = best;
+  });
 }
 
 int pdt_batch_solve(pdt_solver** solvers, int count, int verbose) {

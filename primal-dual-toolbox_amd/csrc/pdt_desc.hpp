@@ -11,19 +11,30 @@
 
 namespace pdt {
 
-constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+constexpr int kBlock = 256;       // threads per workgroup of the stencil kernels (4 wave64)
+constexpr int kRowThreads = 256;  // threads per workgroup of the MRI row FFT passes
+constexpr int kColThreads = 256;  // threads per workgroup of the MRI column pass
 constexpr int kMaxStages = 14;    // FFT stages per transform
 constexpr int kGapTerms = 8;      // partial-sum slots per work item
-constexpr int kMaxRowLen = 1024;  // W limit of the MRI row kernels (4 px / thread)
+constexpr int kMaxRowLen = 4096;  // W limit of the MRI row kernels
+constexpr size_t kLdsMax = 160 * 1024;  // LDS per workgroup on gfx950
 
 // Solver kinds (match PDT_SOLVER_* of pdt_capi.h).
 enum Kind { K_TV = 0, K_TGV = 1, K_TGV_MRI = 2, K_TV_OP = 3, K_TGV_OP = 4 };
 
 // Mixed-radix Stockham plan of one transform length n.
+// Per stage s: radix r, nr = n / r, p = product of the earlier radices,
+// tws = n / (p r) (twiddle stride), and float reciprocals of nr and p for
+// division-free index arithmetic.
 struct FftPlanD {
   int n;
   int ns;
   int r[kMaxStages];
+  int nr[kMaxStages];
+  int p[kMaxStages];
+  int tws[kMaxStages];
+  float rnr[kMaxStages];
+  float rp[kMaxStages];
   const void* tw;  // Cx<R>[n]: exp(-2 pi i m / n)
 };
 
@@ -38,6 +49,7 @@ struct SliceDev {
   int Hk, y0, C;  // k-space FE length, FE offset of the image window, coils
   int ncols;      // listed (sampled) PE columns
   int ngroups;    // column groups of the column pass (per coil)
+  int ngr;        // coil groups of the row passes
   int kind;
   // scalar parameters
   double tau, sigma, theta, lambda;
@@ -67,14 +79,16 @@ struct SliceDev {
   const int* colmap;  // kx -> ks or -1
   void* T1;         // row-pass output [c][y][ks]
   void* T2;         // column-pass output [c][y][ks]
+  void* wpart;      // per-coil-group partial adjoints [g][y][x]
   const void* op_in;  // operator mode input
   void* op_out;       // operator mode output
   FftPlanD planW, planH;
   // control and results
   int* ctl;
   double* stats;
-  // partial-sum ranges of this slice in the row and column work lists
-  int row_beg, row_end, col_beg, col_end;
+  // partial-sum ranges of this slice in the work lists: image rows,
+  // (row, coil group) items and column items
+  int row_beg, row_end, rg_beg, rg_end, col_beg, col_end;
 };
 
 }  // namespace pdt

@@ -107,9 +107,9 @@ template <> struct Ld<Cx<float>, Cx<_Float16>> {
 // ---------------------------------------------------------------------------
 // Block reductions (wave64 shuffles, then the 4 waves in a fixed order)
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int NT = kBlock>
 __device__ __forceinline__ void block_sum(double (&v)[N], double* out) {
-  __shared__ double red[(kBlock / 64) * kGapTerms];
+  __shared__ double red[(NT / 64) * kGapTerms];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int t = 0; t < N; ++t) {
@@ -121,20 +121,21 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double* out) {
   __syncthreads();
   if (threadIdx.x < N) {
     double s = 0;
-    for (int w = 0; w < kBlock / 64; ++w) s += red[w * kGapTerms + threadIdx.x];
+    for (int w = 0; w < NT / 64; ++w) s += red[w * kGapTerms + threadIdx.x];
     out[threadIdx.x] = s;
   }
 }
 
+template <int NT = kBlock>
 __device__ __forceinline__ double block_max(double v) {
-  __shared__ double red[kBlock / 64];
+  __shared__ double red[NT / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
   if (lane == 0) red[wave] = v;
   __syncthreads();
   double m = red[0];
-  for (int w = 1; w < kBlock / 64; ++w) m = fmax(m, red[w]);
+  for (int w = 1; w < NT / 64; ++w) m = fmax(m, red[w]);
   return m;
 }
 
@@ -247,23 +248,28 @@ __device__ void butterfly_gen(const Cx<R>* in, Cx<R>* out, int r, int nr, int p,
   }
 }
 
+// Exact a / b for 0 <= a < 2^24, 1 <= b < 2^12 from a float reciprocal.
+__device__ __forceinline__ int qdiv(int a, int b, float rb) {
+  int q = __float2int_rz(float(a) * rb);
+  if (q * b > a) --q;
+  else if ((q + 1) * b <= a) ++q;
+  return q;
+}
+
 // Transforms nseq sequences of length pl.n stored back to back in a; b is
-// scratch of the same size.  Returns the buffer that holds the result.
-// Ends with a barrier.
-template <class R>
-__device__ Cx<R>* lds_fft(Cx<R>* a, Cx<R>* b, int nseq, const FftPlanD& pl, bool inv) {
+// scratch of the same size; tw is the twiddle table (LDS or global).
+// Returns the buffer that holds the result.  Ends with a barrier.
+template <class R, int NT>
+__device__ Cx<R>* lds_fft(Cx<R>* a, Cx<R>* b, int nseq, const FftPlanD& pl, const Cx<R>* tw, bool inv) {
   const int n = pl.n;
-  const Cx<R>* tw = static_cast<const Cx<R>*>(pl.tw);
-  int p = 1;
   for (int s = 0; s < pl.ns; ++s) {
-    const int r = pl.r[s];
-    const int nr = n / r;
+    const int r = pl.r[s], nr = pl.nr[s], p = pl.p[s], tws = pl.tws[s];
+    const float rnr = pl.rnr[s], rp = pl.rp[s];
     const int total = nseq * nr;
-    const int tws = n / (p * r);
-    for (int t = threadIdx.x; t < total; t += kBlock) {
-      const int seq = t / nr;
+    for (int t = threadIdx.x; t < total; t += NT) {
+      const int seq = qdiv(t, nr, rnr);
       const int i = t - seq * nr;
-      const int k = i % p;
+      const int k = i - qdiv(i, p, rp) * p;
       const Cx<R>* in = a + seq * n + i;
       Cx<R>* out = b + seq * n + (i - k) * r + k;
       const int ts = k * tws;
@@ -280,7 +286,6 @@ __device__ Cx<R>* lds_fft(Cx<R>* a, Cx<R>* b, int nseq, const FftPlanD& pl, bool
     Cx<R>* t = a;
     a = b;
     b = t;
-    p *= r;
   }
   return a;
 }
@@ -597,60 +602,90 @@ __global__ __launch_bounds__(kBlock) void k_tgv_restore2(const SliceDev* sl, con
 // ---------------------------------------------------------------------------
 enum { M_OP = 0, M_TV = 1, M_TGV = 2 };
 
-template <class R, int MODE, bool GAP>
-__global__ __launch_bounds__(kBlock) void k_rows_fwd(const SliceDev* sl, const int2* items, int cb,
-                                                     double* part) {
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Give XCD x a contiguous range of logical work items, so that items
+// touching neighbouring memory (adjacent column groups, adjacent rows) share
+// one L2.  Bijective on [0, n).
+__device__ __forceinline__ int xcd_item(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
+  return x * q + min(x, r) + k;
+}
+
+// LDS layout of the row FFT passes: twiddles (W), two FFT buffers (cb * W
+// each), listed columns (nc ints).
+template <class C>
+struct RowLds {
+  C* tw;
+  C* a;
+  C* b;
+  int* cols;
+  __device__ RowLds(char* raw, int W, int cb) {
+    tw = reinterpret_cast<C*>(raw);
+    a = tw + W;
+    b = a + size_t(cb) * W;
+    cols = reinterpret_cast<int*>(b + size_t(cb) * W);
+  }
+};
+
+// Row pass, forward.  Item = (slice, y * ngr + g): coil group g of image row
+// y.  Group 0 also performs the dual step of row y (TV or TGV).  Then for
+// each coil of the group: S_c * src row -> ifftshift -> PE-FFT -> fftshift
+// -> sampled columns -> T1 [c][y][ks].
+template <class R, int MODE, bool GAP, int NT>
+__global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2* items, int nitems, int cb,
+                                                 double* part) {
   extern __shared__ char smem_raw[];
-  const int2 it = items[blockIdx.x];
+  const int li = xcd_item(blockIdx.x, nitems);
+  const int2 it = items[li];
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using C = Cx<R>;
-  const int W = d.W, H = d.H, y = it.y, C_ = d.C, nc = d.ncols;
-  double t[kGapTerms] = {0};
-  if (MODE != M_OP) {
-    const int k = d.ctl[CTL_ITER];
-    const R sigma = R(d.sigma);
-    const R a0 = sched_alpha<R>(d, d.alpha0, d.alpha0_end, k);
-    const R a1 = sched_alpha<R>(d, d.alpha1, d.alpha1_end, k);
-    for (int x = threadIdx.x; x < W; x += kBlock)
-      dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, a0, a1, t);
-  }
+  const int W = d.W, H = d.H, nc = d.ncols;
+  const int y = it.y / d.ngr, g = it.y - y * d.ngr;
+  const int c0 = g * cb, ncb = min(cb, d.C - c0);
+  RowLds<C> L(smem_raw, W, cb);
+  const C* gtw = static_cast<const C*>(d.planW.tw);
+  for (int e = threadIdx.x; e < W; e += NT) L.tw[e] = gtw[e];
+  for (int e = threadIdx.x; e < nc; e += NT) L.cols[e] = d.cols[e];
   const C* src = static_cast<const C*>(MODE == M_OP ? d.op_in : d.ub) + size_t(y) * W;
   const C* S = static_cast<const C*>(d.S);
   const size_t G = size_t(W) * d.Hk;
   const size_t srow = size_t(y + d.y0) * W;
-  C* T1 = static_cast<C*>(d.T1);
-  C* bufA = reinterpret_cast<C*>(smem_raw);
-  C* bufB = bufA + cb * W;
-  for (int c0 = 0; c0 < C_; c0 += cb) {
-    const int ncb = min(cb, C_ - c0);
-    for (int e = threadIdx.x; e < ncb * W; e += kBlock) {
-      const int cl = e / W, x = e - cl * W;
-      C z = src[x];
-      if (S) z = z * S[G * (c0 + cl) + srow + x];
-      bufA[cl * W + ishift_dst(x, W)] = z;
-    }
-    __syncthreads();
-    C* res = lds_fft<R>(bufA, bufB, ncb, d.planW, false);
-    for (int e = threadIdx.x; e < ncb * nc; e += kBlock) {
-      const int cl = e / nc, ks = e - cl * nc;
-      const int kx = d.cols[ks];
-      T1[(size_t(c0 + cl) * H + y) * nc + ks] = res[cl * W + fshift_src(kx, W)];
-    }
-    __syncthreads();
+  const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
+  for (int e = threadIdx.x; e < ncb * W; e += NT) {
+    const int cl = qdiv(e, W, rW), x = e - cl * W;
+    C z = src[x];
+    if (S) z = z * S[G * (c0 + cl) + srow + x];
+    L.a[cl * W + ishift_dst(x, W)] = z;
   }
-  if (GAP) block_sum<3>(reinterpret_cast<double(&)[3]>(t), part + size_t(blockIdx.x) * kGapTerms);
+  double t[kGapTerms] = {0};
+  if (MODE != M_OP && g == 0) {
+    const int k = d.ctl[CTL_ITER];
+    const R sigma = R(d.sigma);
+    const R a0 = sched_alpha<R>(d, d.alpha0, d.alpha0_end, k);
+    const R a1 = sched_alpha<R>(d, d.alpha1, d.alpha1_end, k);
+    for (int x = threadIdx.x; x < W; x += NT) dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, a0, a1, t);
+  }
+  __syncthreads();
+  C* res = lds_fft<R, NT>(L.a, L.b, ncb, d.planW, L.tw, false);
+  C* T1 = static_cast<C*>(d.T1);
+  for (int e = threadIdx.x; e < ncb * nc; e += NT) {
+    const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
+    T1[(size_t(c0 + cl) * H + y) * nc + ks] = res[cl * W + fshift_src(L.cols[ks], W)];
+  }
+  if (GAP) block_sum<3, NT>(reinterpret_cast<double(&)[3]>(t), part + size_t(li) * kGapTerms);
 }
 
 // Column pass.  CMODE: 0 solver (forward, r prox, adjoint), 1 operator
 // forward (writes op_out {W, Hk, C}), 2 operator adjoint (reads op_in).
 enum { CM_SOLVE = 0, CM_FWD = 1, CM_ADJ = 2 };
 
-template <class R, int CMODE, bool GAP>
-__global__ __launch_bounds__(kBlock) void k_cols(const SliceDev* sl, const int2* items, int cg,
-                                                 double* part) {
+template <class R, int CMODE, bool GAP, int NT>
+__global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* items, int nitems, int cg,
+                                             double* part) {
   extern __shared__ char smem_raw[];
-  const int2 it = items[blockIdx.x];
+  const int li = xcd_item(blockIdx.x, nitems);
+  const int2 it = items[li];
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using C = Cx<R>;
@@ -659,25 +694,29 @@ __global__ __launch_bounds__(kBlock) void k_cols(const SliceDev* sl, const int2*
   const int ks0 = g * cg, ncg = min(cg, nc - ks0);
   const R scale = R(1) / sqrt(R(W) * R(Hk));
   const R* Mc = static_cast<const R*>(d.Mc);
-  C* bufA = reinterpret_cast<C*>(smem_raw);
+  C* tw = reinterpret_cast<C*>(smem_raw);
+  C* bufA = tw + Hk;
   C* bufB = bufA + cg * Hk;
+  const C* gtw = static_cast<const C*>(d.planH.tw);
+  for (int e = threadIdx.x; e < Hk; e += NT) tw[e] = gtw[e];
+  const float rHk = 1.0f / float(Hk), rncg = 1.0f / float(ncg);
   C* res;
   double t[2] = {0, 0};
   if (CMODE != CM_ADJ) {
     const C* T1 = static_cast<const C*>(d.T1);
-    for (int e = threadIdx.x; e < ncg * Hk; e += kBlock) {
-      const int j = e / ncg, cl = e - j * ncg;
+    for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
+      const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
       const int fe = (j + Hk / 2) % Hk;  // ifftshift gather
       const int yy = fe - y0;
       bufA[cl * Hk + j] = (yy >= 0 && yy < H) ? T1[(size_t(c) * H + yy) * nc + ks0 + cl] : pzero<C>();
     }
     __syncthreads();
-    res = lds_fft<R>(bufA, bufB, ncg, d.planH, false);
+    res = lds_fft<R, NT>(bufA, bufB, ncg, d.planH, tw, false);
     C* other = (res == bufA) ? bufB : bufA;
     if (CMODE == CM_FWD) {
       C* out = static_cast<C*>(d.op_out);
-      for (int e = threadIdx.x; e < ncg * Hk; e += kBlock) {
-        const int cl = e / Hk, kfe = e - cl * Hk;
+      for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
+        const int cl = qdiv(e, Hk, rHk), kfe = e - cl * Hk;
         const R m = Mc[size_t(ks0 + cl) * Hk + kfe];
         out[(size_t(c) * Hk + kfe) * W + d.cols[ks0 + cl]] = (m * scale) * res[cl * Hk + fshift_src(kfe, Hk)];
       }
@@ -687,11 +726,11 @@ __global__ __launch_bounds__(kBlock) void k_cols(const SliceDev* sl, const int2*
     const R sigma = R(d.sigma), lambda = R(d.lambda);
     const C* fk = static_cast<const C*>(d.fk);
     C* rk = static_cast<C*>(d.rk);
-    for (int e = threadIdx.x; e < ncg * Hk; e += kBlock) {
-      const int cl = e / Hk, kfe = e - cl * Hk;
+    for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
+      const int cl = qdiv(e, Hk, rHk), kfe = e - cl * Hk;
       const R m = Mc[size_t(ks0 + cl) * Hk + kfe];
-      const C a = (m * scale) * res[cl * Hk + fshift_src(kfe, Hk)];
       const size_t idx = (size_t(c) * nc + ks0 + cl) * Hk + kfe;
+      const C a = (m * scale) * res[cl * Hk + fshift_src(kfe, Hk)];
       const C tmp = a - fk[idx];
       const C rn = (rk[idx] + sigma * tmp) / (R(1) + sigma * lambda);
       rk[idx] = rn;
@@ -702,105 +741,119 @@ __global__ __launch_bounds__(kBlock) void k_cols(const SliceDev* sl, const int2*
       other[cl * Hk + ishift_dst(kfe, Hk)] = m * rn;
     }
     __syncthreads();
-    res = lds_fft<R>(other, res, ncg, d.planH, true);
+    res = lds_fft<R, NT>(other, res, ncg, d.planH, tw, true);
   } else {
     const C* in = static_cast<const C*>(d.op_in);
-    for (int e = threadIdx.x; e < ncg * Hk; e += kBlock) {
-      const int cl = e / Hk, kfe = e - cl * Hk;
+    for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
+      const int cl = qdiv(e, Hk, rHk), kfe = e - cl * Hk;
       const R m = Mc[size_t(ks0 + cl) * Hk + kfe];
       bufA[cl * Hk + ishift_dst(kfe, Hk)] = m * in[(size_t(c) * Hk + kfe) * W + d.cols[ks0 + cl]];
     }
     __syncthreads();
-    res = lds_fft<R>(bufA, bufB, ncg, d.planH, true);
+    res = lds_fft<R, NT>(bufA, bufB, ncg, d.planH, tw, true);
   }
   C* T2 = static_cast<C*>(d.T2);
-  for (int e = threadIdx.x; e < ncg * H; e += kBlock) {
-    const int yy = e / ncg, cl = e - yy * ncg;
+  for (int e = threadIdx.x; e < ncg * H; e += NT) {
+    const int yy = qdiv(e, ncg, rncg), cl = e - yy * ncg;
     const int fe = yy + y0;
     T2[(size_t(c) * H + yy) * nc + ks0 + cl] = scale * res[cl * Hk + fshift_src(fe, Hk)];
   }
-  if (GAP) block_sum<2>(t, part + size_t(blockIdx.x) * kGapTerms);
+  if (GAP) block_sum<2, NT>(t, part + size_t(li) * kGapTerms);
 }
 
-template <class R, int MODE, bool GAP>
-__global__ __launch_bounds__(kBlock) void k_rows_adj(const SliceDev* sl, const int2* items, int cb,
-                                                     double* part) {
+// Row pass, adjoint.  Item = (slice, y * ngr + g): for each coil of group g,
+// T2 row -> ifftshift -> inverse PE-FFT -> fftshift -> * conj(S_c); the
+// group's coils are summed in order into wpart[g][y][x].
+template <class R, int NT>
+__global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2* items, int nitems, int cb) {
   extern __shared__ char smem_raw[];
-  const int2 it = items[blockIdx.x];
+  const int li = xcd_item(blockIdx.x, nitems);
+  const int2 it = items[li];
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using C = Cx<R>;
-  const int W = d.W, H = d.H, y = it.y, C_ = d.C, nc = d.ncols;
+  const int W = d.W, H = d.H, nc = d.ncols;
+  const int y = it.y / d.ngr, g = it.y - y * d.ngr;
+  const int c0 = g * cb, ncb = min(cb, d.C - c0);
+  RowLds<C> L(smem_raw, W, cb);
+  const C* gtw = static_cast<const C*>(d.planW.tw);
+  for (int e = threadIdx.x; e < W; e += NT) L.tw[e] = gtw[e];
+  for (int e = threadIdx.x; e < ncb * W; e += NT) L.a[e] = pzero<C>();
+  for (int e = threadIdx.x; e < nc; e += NT) L.cols[e] = d.cols[e];
+  __syncthreads();
+  const C* T2 = static_cast<const C*>(d.T2);
+  const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
+  for (int e = threadIdx.x; e < ncb * nc; e += NT) {
+    const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
+    L.a[cl * W + ishift_dst(L.cols[ks], W)] = T2[(size_t(c0 + cl) * H + y) * nc + ks];
+  }
+  __syncthreads();
+  C* res = lds_fft<R, NT>(L.a, L.b, ncb, d.planW, L.tw, true);
+  C* other = (res == L.a) ? L.b : L.a;
   const C* S = static_cast<const C*>(d.S);
   const size_t G = size_t(W) * d.Hk;
   const size_t srow = size_t(y + d.y0) * W;
-  const C* T2 = static_cast<const C*>(d.T2);
-  C* bufA = reinterpret_cast<C*>(smem_raw);
-  C* bufB = bufA + cb * W;
-  constexpr int NX = kMaxRowLen / kBlock;
-  C acc[NX];
-#pragma unroll
-  for (int j = 0; j < NX; ++j) acc[j] = pzero<C>();
-  for (int c0 = 0; c0 < C_; c0 += cb) {
-    const int ncb = min(cb, C_ - c0);
-    for (int e = threadIdx.x; e < ncb * W; e += kBlock) {
-      const int cl = e / W, j = e - cl * W;
-      const int kx = (j + W / 2) % W;  // ifftshift gather
-      const int ks = d.colmap[kx];
-      bufA[cl * W + j] = ks >= 0 ? T2[(size_t(c0 + cl) * H + y) * nc + ks] : pzero<C>();
-    }
-    __syncthreads();
-    C* res = lds_fft<R>(bufA, bufB, ncb, d.planW, true);
-#pragma unroll
-    for (int j = 0; j < NX; ++j) {
-      const int x = threadIdx.x + j * kBlock;
-      if (x < W) {
-        const int src = fshift_src(x, W);
-        for (int cl = 0; cl < ncb; ++cl) {
-          const C v = res[cl * W + src];
-          acc[j] = acc[j] + (S ? mulconj(v, S[G * (c0 + cl) + srow + x]) : v);
-        }
-      }
-    }
-    __syncthreads();
+  for (int e = threadIdx.x; e < ncb * W; e += NT) {
+    const int cl = qdiv(e, W, rW), x = e - cl * W;
+    const C v = res[cl * W + fshift_src(x, W)];
+    other[cl * W + x] = S ? mulconj(v, S[G * (c0 + cl) + srow + x]) : v;
   }
-  if (MODE == M_OP) {
-    C* out = static_cast<C*>(d.op_out) + size_t(y) * W;
-#pragma unroll
-    for (int j = 0; j < NX; ++j) {
-      const int x = threadIdx.x + j * kBlock;
-      if (x < W) out[x] = acc[j];
-    }
-    return;
+  __syncthreads();
+  C* wp = static_cast<C*>(d.wpart) + (size_t(g) * H + y) * W;
+  for (int x = threadIdx.x; x < W; x += NT) {
+    C acc = other[x];
+    for (int cl = 1; cl < ncb; ++cl) acc = acc + other[cl * W + x];
+    wp[x] = acc;
   }
-  // TGV_primal_u_noprox_kernel (tgv_kernels.cuh:432-467) with w = A^* r,
-  // then TGV_primal_v_kernel (:647-686).
-  const R tau = R(d.tau), theta = R(d.theta);
+}
+
+// Primal half of the MRI solvers: w = sum_g wpart[g] (A^* r), then
+// TGV_primal_u_noprox_kernel (tgv_kernels.cuh:432-467) and, for TGV,
+// TGV_primal_v_kernel (:647-686).  Item = (slice, first row), rpi rows.
+// MODE M_OP writes w to op_out instead (operator adjoint).
+template <class R, int MODE, bool GAP>
+__global__ __launch_bounds__(kBlock) void k_mri_primal(const SliceDev* sl, const int2* items, int nitems, int rpi,
+                                                       double* part) {
+  const int li = xcd_item(blockIdx.x, nitems);
+  const int2 it = items[li];
+  const SliceDev& d = sl[it.x];
+  if (!slice_active(d)) return;
+  using C = Cx<R>;
+  const int W = d.W, H = d.H, ngr = d.ngr;
   const size_t n = size_t(W) * H;
+  const int ya = it.y, yb = min(H, ya + rpi);
+  const C* wp = static_cast<const C*>(d.wpart);
+  const R tau = R(d.tau), theta = R(d.theta);
   C* u = static_cast<C*>(d.u);
   C* ub = static_cast<C*>(d.ub);
   const C* px = static_cast<const C*>(d.p);
   const C* py = px + n;
   double t[2] = {0, 0};
-#pragma unroll
-  for (int j = 0; j < NX; ++j) {
-    const int x = threadIdx.x + j * kBlock;
-    if (x < W) {
-      const size_t i = size_t(y) * W + x;
-      const C uo = u[i];
-      const C div = dmx<C>(px, x, i, W) + dmy<C>(py, y, i, W, H);
-      const C un = uo + tau * (div - acc[j]);
-      u[i] = un;
-      ub[i] = (R(1) + theta) * un - theta * uo;
-      if (GAP) t[0] += dabs2(acc[j] - div);
-      if (MODE == M_TGV) {
-        const double rv = vstep_pixel<C, C>(d, x, y, tau, theta);
-        if (GAP) t[1] += rv;
-      }
+  const int cnt = (yb - ya) * W;
+  for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
+    const int yy = idx / W;
+    const int x = idx - yy * W, y = ya + yy;
+    const size_t i = size_t(y) * W + x;
+    C w = wp[i];
+    for (int g = 1; g < ngr; ++g) w = w + wp[size_t(g) * n + i];
+    if (MODE == M_OP) {
+      static_cast<C*>(d.op_out)[i] = w;
+      continue;
+    }
+    const C uo = u[i];
+    const C div = dmx<C>(px, x, i, W) + dmy<C>(py, y, i, W, H);
+    const C un = uo + tau * (div - w);
+    u[i] = un;
+    ub[i] = (R(1) + theta) * un - theta * uo;
+    if (GAP) t[0] += dabs2(w - div);
+    if (MODE == M_TGV) {
+      const double rv = vstep_pixel<C, C>(d, x, y, tau, theta);
+      if (GAP) t[1] += rv;
     }
   }
-  if (GAP) block_sum<2>(t, part + size_t(blockIdx.x) * kGapTerms);
-  if (y == 0 && threadIdx.x == 0) {
+  if (MODE == M_OP) return;
+  if (GAP) block_sum<2>(t, part + size_t(li) * kGapTerms);
+  if (ya == 0 && threadIdx.x == 0) {
     d.ctl[CTL_ITER] += 1;
     d.ctl[CTL_RUPD] += 1;
   }
@@ -821,6 +874,14 @@ __global__ __launch_bounds__(kBlock) void k_gap_reduce(const SliceDev* sl, int k
   const bool mri = kind == K_TGV_MRI || kind == K_TV_OP || kind == K_TGV_OP;
   double s[kGapTerms] = {0};
   double mx = 0;
+  if (mri) {
+    for (int i = d.rg_beg + threadIdx.x; i < d.rg_end; i += kBlock) {
+      const double* a = p0 + size_t(i) * kGapTerms;
+      s[0] += a[0];
+      s[1] += a[1];
+      s[2] += a[2];
+    }
+  }
   for (int i = d.row_beg + threadIdx.x; i < d.row_end; i += kBlock) {
     const double* a = p0 + size_t(i) * kGapTerms;
     if (kind == K_TV) {
@@ -836,9 +897,6 @@ __global__ __launch_bounds__(kBlock) void k_gap_reduce(const SliceDev* sl, int k
       s[4] += p3[size_t(i) * kGapTerms + 0];
       s[5] += p3[size_t(i) * kGapTerms + 1];
     } else {
-      s[0] += a[0];
-      s[1] += a[1];
-      s[2] += a[2];
       s[5] += p2[size_t(i) * kGapTerms + 0];
       s[6] += p2[size_t(i) * kGapTerms + 1];
     }
@@ -941,30 +999,43 @@ __global__ __launch_bounds__(kBlock) void k_zero(uint32_t* p, size_t n) {
 // ---------------------------------------------------------------------------
 #define PDT_LAUNCH(kern, grid, lds, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, __VA_ARGS__)
 
-size_t lds_rows_bytes(int dtype, int cb, int W) {
+size_t lds_rows_bytes(int dtype, int cb, int W, int nc) {
   const size_t e = (dtype == DT_C128 || dtype == DT_F64) ? 16 : 8;
-  return 2 * size_t(cb) * W * e;
+  return (size_t(W) + 2 * size_t(cb) * W) * e + size_t(nc) * sizeof(int);
 }
 size_t lds_cols_bytes(int dtype, int cg, int Hk) {
   const size_t e = (dtype == DT_C128 || dtype == DT_F64) ? 16 : 8;
-  return 2 * size_t(cg) * Hk * e;
+  return (size_t(Hk) + 2 * size_t(cg) * Hk) * e;
 }
 
+int num_stages(int kind) { return (kind == K_TV || kind == K_TGV) ? 2 : 4; }
+
+const char* stage_name(int kind, int stage) {
+  switch (kind) {
+    case K_TV: return stage == 0 ? "k_tv_dual" : "k_tv_primal";
+    case K_TGV: return stage == 0 ? "k_tgv_primal" : "k_tgv_dual";
+    default: return stage == 0 ? "k_rows_fwd" : stage == 1 ? "k_cols" : stage == 2 ? "k_rows_adj" : "k_mri_primal";
+  }
+}
+
+// Denoising stages: TV = {dual, primal}; TGV = {primal, dual (+ restoration
+// on gap iterations)}.
 template <class P, class S>
-static void denoise_iteration(const Launch& L, int kind, bool gap) {
+static void denoise_stage(const Launch& L, int kind, bool gap, int stage) {
   const int n = L.nrows;
   if (n == 0) return;
   if (kind == K_TV) {
-    if (gap) {
-      PDT_LAUNCH((k_tv_dual<P, true>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[0]);
-      PDT_LAUNCH((k_tv_primal<P, true>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[1]);
+    if (stage == 0) {
+      if (gap) PDT_LAUNCH((k_tv_dual<P, true>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[0]);
+      else PDT_LAUNCH((k_tv_dual<P, false>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[0]);
     } else {
-      PDT_LAUNCH((k_tv_dual<P, false>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[0]);
-      PDT_LAUNCH((k_tv_primal<P, false>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[1]);
+      if (gap) PDT_LAUNCH((k_tv_primal<P, true>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[1]);
+      else PDT_LAUNCH((k_tv_primal<P, false>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[1]);
     }
   } else {
-    PDT_LAUNCH((k_tgv_primal<P, S>), n, 0, L.stream, L.sl, L.rows, L.rpi);
-    if (gap) {
+    if (stage == 0) {
+      PDT_LAUNCH((k_tgv_primal<P, S>), n, 0, L.stream, L.sl, L.rows, L.rpi);
+    } else if (gap) {
       PDT_LAUNCH((k_tgv_dual<P, S, true>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[0]);
       PDT_LAUNCH((k_tgv_restore1<P, S>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[2]);
       PDT_LAUNCH((k_tgv_restore2<P>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[3]);
@@ -974,50 +1045,56 @@ static void denoise_iteration(const Launch& L, int kind, bool gap) {
   }
 }
 
+#define PDT_LAUNCH_NT(kern, grid, nt, lds, st, ...) \
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(nt), lds, st, __VA_ARGS__)
+
+// MRI stages: 0 rows_fwd, 1 cols, 2 rows_adj, 3 primal.
+template <class R, int MODE>
+static void mri_stage_m(const Launch& L, bool gap, int stage) {
+  constexpr int NR = kRowThreads, NC = kColThreads;
+  if (stage == 0) {
+    if (gap) PDT_LAUNCH_NT((k_rows_fwd<R, MODE, true, NR>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0]);
+    else PDT_LAUNCH_NT((k_rows_fwd<R, MODE, false, NR>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0]);
+  } else if (stage == 1) {
+    if (L.ncolitems == 0) return;
+    if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+    else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+  } else if (stage == 2) {
+    PDT_LAUNCH_NT((k_rows_adj<R, NR>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb);
+  } else {
+    if (gap) PDT_LAUNCH((k_mri_primal<R, MODE, true>), L.nrows, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
+    else PDT_LAUNCH((k_mri_primal<R, MODE, false>), L.nrows, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
+  }
+}
+
 template <class R>
-static void mri_iteration(const Launch& L, int kind, bool gap) {
-  const size_t ldsr = L.lds_rows, ldsc = L.lds_cols;
-  if (kind == K_TV_OP) {
-    if (gap) {
-      PDT_LAUNCH((k_rows_fwd<R, M_TV, true>), L.nrows, ldsr, L.stream, L.sl, L.rows, L.cb, L.part[0]);
-      PDT_LAUNCH((k_cols<R, CM_SOLVE, true>), L.ncolitems, ldsc, L.stream, L.sl, L.cols, L.cg, L.part[1]);
-      PDT_LAUNCH((k_rows_adj<R, M_TV, true>), L.nrows, ldsr, L.stream, L.sl, L.rows, L.cb, L.part[2]);
-    } else {
-      PDT_LAUNCH((k_rows_fwd<R, M_TV, false>), L.nrows, ldsr, L.stream, L.sl, L.rows, L.cb, L.part[0]);
-      PDT_LAUNCH((k_cols<R, CM_SOLVE, false>), L.ncolitems, ldsc, L.stream, L.sl, L.cols, L.cg, L.part[1]);
-      PDT_LAUNCH((k_rows_adj<R, M_TV, false>), L.nrows, ldsr, L.stream, L.sl, L.rows, L.cb, L.part[2]);
+static void mri_stage(const Launch& L, int kind, bool gap, int stage) {
+  if (kind == K_TV_OP) mri_stage_m<R, M_TV>(L, gap, stage);
+  else mri_stage_m<R, M_TGV>(L, gap, stage);
+}
+
+void run_stage(const Launch& L, int kind, int dtype, bool fp16, bool gap, int stage) {
+  if (kind == K_TV || kind == K_TGV) {
+    switch (dtype) {
+      case DT_F32:
+        if (fp16) denoise_stage<float, _Float16>(L, kind, gap, stage);
+        else denoise_stage<float, float>(L, kind, gap, stage);
+        break;
+      case DT_C64:
+        if (fp16) denoise_stage<Cx<float>, Cx<_Float16>>(L, kind, gap, stage);
+        else denoise_stage<Cx<float>, Cx<float>>(L, kind, gap, stage);
+        break;
+      case DT_F64: denoise_stage<double, double>(L, kind, gap, stage); break;
+      case DT_C128: denoise_stage<Cx<double>, Cx<double>>(L, kind, gap, stage); break;
     }
   } else {
-    if (gap) {
-      PDT_LAUNCH((k_rows_fwd<R, M_TGV, true>), L.nrows, ldsr, L.stream, L.sl, L.rows, L.cb, L.part[0]);
-      PDT_LAUNCH((k_cols<R, CM_SOLVE, true>), L.ncolitems, ldsc, L.stream, L.sl, L.cols, L.cg, L.part[1]);
-      PDT_LAUNCH((k_rows_adj<R, M_TGV, true>), L.nrows, ldsr, L.stream, L.sl, L.rows, L.cb, L.part[2]);
-    } else {
-      PDT_LAUNCH((k_rows_fwd<R, M_TGV, false>), L.nrows, ldsr, L.stream, L.sl, L.rows, L.cb, L.part[0]);
-      PDT_LAUNCH((k_cols<R, CM_SOLVE, false>), L.ncolitems, ldsc, L.stream, L.sl, L.cols, L.cg, L.part[1]);
-      PDT_LAUNCH((k_rows_adj<R, M_TGV, false>), L.nrows, ldsr, L.stream, L.sl, L.rows, L.cb, L.part[2]);
-    }
+    if (dtype == DT_C64) mri_stage<float>(L, kind, gap, stage);
+    else mri_stage<double>(L, kind, gap, stage);
   }
 }
 
 void run_iteration(const Launch& L, int kind, int dtype, bool fp16, bool gap) {
-  if (kind == K_TV || kind == K_TGV) {
-    switch (dtype) {
-      case DT_F32:
-        if (fp16) denoise_iteration<float, _Float16>(L, kind, gap);
-        else denoise_iteration<float, float>(L, kind, gap);
-        break;
-      case DT_C64:
-        if (fp16) denoise_iteration<Cx<float>, Cx<_Float16>>(L, kind, gap);
-        else denoise_iteration<Cx<float>, Cx<float>>(L, kind, gap);
-        break;
-      case DT_F64: denoise_iteration<double, double>(L, kind, gap); break;
-      case DT_C128: denoise_iteration<Cx<double>, Cx<double>>(L, kind, gap); break;
-    }
-  } else {
-    if (dtype == DT_C64) mri_iteration<float>(L, kind, gap);
-    else mri_iteration<double>(L, kind, gap);
-  }
+  for (int s = 0; s < num_stages(kind); ++s) run_stage(L, kind, dtype, fp16, gap, s);
 }
 
 void run_gap_reduce(const Launch& L, int kind, int dtype) {
@@ -1027,24 +1104,57 @@ void run_gap_reduce(const Launch& L, int kind, int dtype) {
     PDT_LAUNCH((k_gap_reduce<double>), L.nsl, 0, L.stream, L.sl, kind, L.part[0], L.part[1], L.part[2], L.part[3]);
 }
 
-void run_op_forward(const Launch& L, int dtype) {
-  if (dtype == DT_C64) {
-    PDT_LAUNCH((k_rows_fwd<float, M_OP, false>), L.nrows, L.lds_rows, L.stream, L.sl, L.rows, L.cb, L.part[0]);
-    PDT_LAUNCH((k_cols<float, CM_FWD, false>), L.ncolitems, L.lds_cols, L.stream, L.sl, L.cols, L.cg, L.part[1]);
+template <class R>
+static void op_forward(const Launch& L) {
+  constexpr int NR = kRowThreads, NC = kColThreads;
+  PDT_LAUNCH_NT((k_rows_fwd<R, M_OP, false, NR>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0]);
+  PDT_LAUNCH_NT((k_cols<R, CM_FWD, false, NC>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+}
+
+template <class R>
+static void op_adjoint(const Launch& L) {
+  constexpr int NR = kRowThreads, NC = kColThreads;
+  PDT_LAUNCH_NT((k_cols<R, CM_ADJ, false, NC>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+  PDT_LAUNCH_NT((k_rows_adj<R, NR>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb);
+  PDT_LAUNCH((k_mri_primal<R, M_OP, false>), L.nrows, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
+}
+
+template <class R>
+static bool set_lds_attrs(int b) {
+  constexpr int NR = kRowThreads, NC = kColThreads;
+  const void* fns[] = {
+      (const void*)&k_rows_fwd<R, M_OP, false, NR>, (const void*)&k_rows_fwd<R, M_TV, false, NR>,
+      (const void*)&k_rows_fwd<R, M_TV, true, NR>,  (const void*)&k_rows_fwd<R, M_TGV, false, NR>,
+      (const void*)&k_rows_fwd<R, M_TGV, true, NR>, (const void*)&k_rows_adj<R, NR>,
+      (const void*)&k_cols<R, CM_SOLVE, false, NC>, (const void*)&k_cols<R, CM_SOLVE, true, NC>,
+      (const void*)&k_cols<R, CM_FWD, false, NC>,   (const void*)&k_cols<R, CM_ADJ, false, NC>};
+  bool ok = true;
+  for (const void* f : fns) ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, b) == hipSuccess;
+  return ok;
+}
+
+size_t init_kernel_attrs() {
+  static size_t budget = 0;
+  if (budget) return budget;
+  // the block reductions use up to 2 KiB of static LDS
+  const size_t want = kLdsMax - 4096;
+  if (set_lds_attrs<float>(int(want)) && set_lds_attrs<double>(int(want))) {
+    budget = want;
   } else {
-    PDT_LAUNCH((k_rows_fwd<double, M_OP, false>), L.nrows, L.lds_rows, L.stream, L.sl, L.rows, L.cb, L.part[0]);
-    PDT_LAUNCH((k_cols<double, CM_FWD, false>), L.ncolitems, L.lds_cols, L.stream, L.sl, L.cols, L.cg, L.part[1]);
+    budget = 64 * 1024 - 4096;
   }
+  (void)hipGetLastError();  // a refused attribute must not surface at the next launch check
+  return budget;
+}
+
+void run_op_forward(const Launch& L, int dtype) {
+  if (dtype == DT_C64) op_forward<float>(L);
+  else op_forward<double>(L);
 }
 
 void run_op_adjoint(const Launch& L, int dtype) {
-  if (dtype == DT_C64) {
-    PDT_LAUNCH((k_cols<float, CM_ADJ, false>), L.ncolitems, L.lds_cols, L.stream, L.sl, L.cols, L.cg, L.part[1]);
-    PDT_LAUNCH((k_rows_adj<float, M_OP, false>), L.nrows, L.lds_rows, L.stream, L.sl, L.rows, L.cb, L.part[2]);
-  } else {
-    PDT_LAUNCH((k_cols<double, CM_ADJ, false>), L.ncolitems, L.lds_cols, L.stream, L.sl, L.cols, L.cg, L.part[1]);
-    PDT_LAUNCH((k_rows_adj<double, M_OP, false>), L.nrows, L.lds_rows, L.stream, L.sl, L.rows, L.cb, L.part[2]);
-  }
+  if (dtype == DT_C64) op_adjoint<float>(L);
+  else op_adjoint<double>(L);
 }
 
 int run_compact_data(const Launch& L, int dtype, const void* f_full, double* f0_part) {

@@ -16,11 +16,13 @@ struct Launch {
   int nsl = 0;
   const int2* rows = nullptr;    // (slice, first row) items
   int nrows = 0;
+  const int2* rowgroups = nullptr;  // (slice, row * ngr + coil group) items (MRI row FFT passes)
+  int nrg = 0;
   int rpi = 1;                   // rows per item (denoising kernels)
   const int2* cols = nullptr;    // (slice, coil * ngroups + group) items
   int ncolitems = 0;
   double* part[4] = {nullptr, nullptr, nullptr, nullptr};
-  int cb = 1;                    // coils per LDS chunk in the row kernels
+  int cb = 1;                    // coils per row work item (coil group size)
   int cg = 1;                    // columns per group in the column kernel
   size_t lds_rows = 0;           // dynamic LDS bytes of the row kernels
   size_t lds_cols = 0;           // dynamic LDS bytes of the column kernel
@@ -28,6 +30,10 @@ struct Launch {
 
 // One full iteration of the solver kind on every active slice.
 void run_iteration(const Launch& L, int kind, int dtype, bool fp16, bool gap);
+// One stage (fused kernel) of an iteration; stages run in order 0..n-1.
+int num_stages(int kind);
+const char* stage_name(int kind, int stage);
+void run_stage(const Launch& L, int kind, int dtype, bool fp16, bool gap, int stage);
 // Gap evaluation from the partial sums of the last iteration (one block/slice).
 void run_gap_reduce(const Launch& L, int kind, int dtype);
 // Operator application in operator mode (op_in -> op_out of every slice).
@@ -40,8 +46,12 @@ int run_compact_data(const Launch& L, int dtype, const void* f_full, double* f0_
 // Fills n elements of the given size (bytes per element 4, 8 or 16) with 0.
 void run_zero(hipStream_t st, void* ptr, size_t bytes);
 
+// Allows the MRI kernels the full LDS of a workgroup (call before launches);
+// returns the dynamic LDS bytes a workgroup may use.
+size_t init_kernel_attrs();
+
 // LDS bytes needed by the row / column kernels.
-size_t lds_rows_bytes(int dtype, int cb, int W);
+size_t lds_rows_bytes(int dtype, int cb, int W, int nc);
 size_t lds_cols_bytes(int dtype, int cg, int Hk);
 
 }  // namespace pdt

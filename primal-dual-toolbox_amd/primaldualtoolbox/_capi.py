@@ -29,7 +29,7 @@ EXPORTS = [
     "pdt_solver_create", "pdt_params_default", "pdt_solver_get_params", "pdt_solver_set_params",
     "pdt_solver_set_operator", "pdt_solver_set_noisy_data", "pdt_solver_set_input0",
     "pdt_solver_solve", "pdt_solver_get_result", "pdt_solver_get_v", "pdt_solver_get_stats",
-    "pdt_solver_destroy", "pdt_batch_solve",
+    "pdt_solver_destroy", "pdt_solver_profile", "pdt_autotune", "pdt_batch_solve",
 ]
 
 
@@ -44,6 +44,10 @@ class PdtStats(ct.Structure):
     _fields_ = [("iters", ct.c_uint32), ("n_gap", ct.c_uint32), ("primal", ct.c_double),
                 ("dual", ct.c_double), ("gap", ct.c_double), ("rel_gap", ct.c_double),
                 ("ms", ct.c_double), ("converged", ct.c_int32)]
+
+
+class PdtKernelProf(ct.Structure):
+    _fields_ = [("name", ct.c_char * 32), ("ms", ct.c_double), ("bytes", ct.c_double)]
 
 
 class PdtError(RuntimeError):
@@ -96,6 +100,10 @@ def lib():
         L.pdt_solver_get_v.argtypes = [vp, ct.POINTER(vp)]
         L.pdt_solver_get_stats.argtypes = [vp, ct.POINTER(PdtStats)]
         L.pdt_solver_destroy.argtypes = [vp]
+        L.pdt_solver_profile.argtypes = [vp, ct.c_int, ct.POINTER(PdtKernelProf), ct.c_int,
+                                         ct.POINTER(ct.c_int)]
+        ip = ct.POINTER(ct.c_int)
+        L.pdt_autotune.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ip, ip, ip, dp]
         L.pdt_batch_solve.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int]
         _lib = L
     return _lib

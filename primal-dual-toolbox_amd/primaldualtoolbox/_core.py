@@ -231,10 +231,33 @@ class _Optimizer:
         C.check(C.lib().pdt_solver_get_stats(self._h, ct.byref(st)))
         return {k: getattr(st, k) for k, _ in C.PdtStats._fields_}
 
+    def profileKernels(self, reps: int = 20) -> list:
+        """Per fused kernel: name, average ms per launch (hipEvents on the
+        solver stream) and algorithmic bytes per launch.  Advances the state
+        arbitrarily, so call it after the solves whose results are used."""
+        self._push_params()
+        out = (C.PdtKernelProf * 8)()
+        n = ct.c_int()
+        C.check(C.lib().pdt_solver_profile(self._h, int(reps), out, 8, ct.byref(n)))
+        return [{"name": out[i].name.decode(), "ms": out[i].ms, "bytes": out[i].bytes} for i in range(n.value)]
+
     def __del__(self):
         if getattr(self, "_h", None) and C._lib is not None:
             C._lib.pdt_solver_destroy(self._h)
             self._h = None
+
+
+def autotune(optimizers, reps: int = 20) -> dict:
+    """Times the candidate tile shapes on these configured optimizers (one
+    batch), caches the fastest and returns it.  Leaves the state advanced:
+    reset with setInput0 / setNoisyData before solving."""
+    for o in optimizers:
+        o._push_params()
+    arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
+    cb, cg, rpi, ms = ct.c_int(), ct.c_int(), ct.c_int(), ct.c_double()
+    C.check(C.lib().pdt_autotune(arr, len(optimizers), int(reps), ct.byref(cb), ct.byref(cg), ct.byref(rpi),
+                                 ct.byref(ms)))
+    return {"cb": cb.value, "cg": cg.value, "rpi": rpi.value, "ms_per_iteration": ms.value}
 
 
 def batch_solve(optimizers, verbose: bool = False) -> None:
