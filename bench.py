@@ -90,11 +90,26 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=50)
+    ap.add_argument("--fixed-iters", type=int, default=0,
+                    help="profiling mode: run this many iterations per step (no gap stop), print no JSON")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="profiling mode: run a plain device copy of this many MiB (PMC byte calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(slices shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {args.calibrate_copy} MiB x 5")
+        return
 
     from primaldualtoolbox import _capi as C
 
@@ -110,6 +125,15 @@ def main():
     import primaldualtoolbox as pdt
     tune = pdt.mri.autotune([opt], reps=20)
     log(f"[bench] autotuned tiles: {tune}")
+    if args.fixed_iters:
+        prm = opt.getParameters()
+        prm.max_iter, prm.gap_tol = args.fixed_iters - 1, 0.0
+        for k in range(max(args.steps, 1)):
+            opt.setNoisyData(case["kspace"])
+            cold_start(opt, u0_dev)
+            opt.solve()
+            log(f"[bench] fixed step {k}: {opt.getStats()}")
+        return
 
     iters = []
     gaps = []

@@ -223,11 +223,9 @@ template <class R, int RR>
 __device__ __forceinline__ void butterfly(const Cx<R>* in, Cx<R>* out, int nr, int p,
                                           const Cx<R>* tw, int ts, bool inv) {
   Cx<R> x[RR];
+  x[0] = in[0];
 #pragma unroll
-  for (int j = 0; j < RR; ++j) {
-    x[j] = in[j * nr];
-    if (j > 0 && ts > 0) x[j] = x[j] * twd(tw, j * ts, inv);
-  }
+  for (int j = 1; j < RR; ++j) x[j] = in[j * nr] * twd(tw, j * ts, inv);
   Dft<R, RR>::run(x, inv ? R(1) : R(-1));
 #pragma unroll
   for (int q = 0; q < RR; ++q) out[q * p] = x[q];
@@ -241,7 +239,7 @@ __device__ void butterfly_gen(const Cx<R>* in, Cx<R>* out, int r, int nr, int p,
     Cx<R> acc = pzero<Cx<R>>();
     for (int j = 0; j < r; ++j) {
       Cx<R> v = in[j * nr];
-      if (j > 0 && ts > 0) v = v * twd(tw, j * ts, inv);
+      if (j > 0) v = v * twd(tw, j * ts, inv);
       acc = acc + v * twd(tw, ((j * q) % r) * nr, inv);
     }
     out[q * p] = acc;
@@ -265,11 +263,12 @@ __device__ Cx<R>* lds_fft(Cx<R>* a, Cx<R>* b, int nseq, const FftPlanD& pl, cons
   for (int s = 0; s < pl.ns; ++s) {
     const int r = pl.r[s], nr = pl.nr[s], p = pl.p[s], tws = pl.tws[s];
     const float rnr = pl.rnr[s], rp = pl.rp[s];
+    const bool p2 = (p & (p - 1)) == 0;  // uniform: power-of-two radices come first
     const int total = nseq * nr;
     for (int t = threadIdx.x; t < total; t += NT) {
-      const int seq = qdiv(t, nr, rnr);
+      const int seq = nseq == 1 ? 0 : qdiv(t, nr, rnr);
       const int i = t - seq * nr;
-      const int k = i - qdiv(i, p, rp) * p;
+      const int k = p2 ? (i & (p - 1)) : i - qdiv(i, p, rp) * p;
       const Cx<R>* in = a + seq * n + i;
       Cx<R>* out = b + seq * n + (i - k) * r + k;
       const int ts = k * tws;
@@ -290,11 +289,108 @@ __device__ Cx<R>* lds_fft(Cx<R>* a, Cx<R>* b, int nseq, const FftPlanD& pl, cons
   return a;
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time-planned Stockham FFT for the common transform lengths.  Same
+// factorisation and stage formula as lds_fft, with radix, stride and twiddle
+// steps constant-folded; lengths without a specialisation use lds_fft.
+// ---------------------------------------------------------------------------
+constexpr int ct_radix(int n, int s) {
+  int k = 0;
+  while (n % 8 == 0 && n > 8) {
+    if (k == s) return 8;
+    n /= 8;
+    ++k;
+  }
+  const int small[5] = {8, 4, 2, 3, 5};
+  for (int j = 0; j < 5; ++j)
+    while (n % small[j] == 0 && n >= small[j]) {
+      if (k == s) return small[j];
+      n /= small[j];
+      ++k;
+    }
+  for (int p = 7; n > 1; p += 2)
+    while (n % p == 0) {
+      if (k == s) return p;
+      n /= p;
+      ++k;
+    }
+  return 1;
+}
+constexpr int ct_nstages(int n) {
+  int s = 0;
+  while (ct_radix(n, s) != 1) ++s;
+  return s;
+}
+
+template <class R, int RR>
+__device__ __forceinline__ void butterfly_ct(const Cx<R>* in, Cx<R>* out, int ts, const Cx<R>* tw, bool inv,
+                                             int nr, int p) {
+  if constexpr (RR == 2 || RR == 3 || RR == 4 || RR == 5 || RR == 8) {
+    Cx<R> x[RR];
+    x[0] = in[0];
+#pragma unroll
+    for (int j = 1; j < RR; ++j) x[j] = in[j * nr] * twd(tw, j * ts, inv);
+    Dft<R, RR>::run(x, inv ? R(1) : R(-1));
+#pragma unroll
+    for (int q = 0; q < RR; ++q) out[q * p] = x[q];
+  } else {
+    // odd prime radix: direct DFT reading its inputs from LDS (keeps the
+    // register footprint of the kernel small)
+    butterfly_gen<R>(in, out, RR, nr, p, tw, ts, inv);
+  }
+}
+
+template <class R, int N, int S, int P, int NT, bool INV>
+__device__ __forceinline__ Cx<R>* fft_ct_stage(Cx<R>* a, Cx<R>* b, int nseq, const Cx<R>* tw) {
+  if constexpr (S == ct_nstages(N)) {
+    return a;
+  } else {
+    constexpr int r = ct_radix(N, S), nr = N / r, tws = N / (P * r);
+    const int total = nseq * nr;
+    for (int t = threadIdx.x; t < total; t += NT) {
+      const int seq = t / nr;
+      const int i = t - seq * nr;
+      const int k = i % P;
+      butterfly_ct<R, r>(a + seq * N + i, b + seq * N + (i - k) * r + k, k * tws, tw, INV, nr, P);
+    }
+    __syncthreads();
+    return fft_ct_stage<R, N, S + 1, P * r, NT, INV>(b, a, nseq, tw);
+  }
+}
+
+// Lengths of the benchmark workloads: PE 256/320/368/384, FE 512/640/736/768.
+template <class R, int NT, bool INV>
+__device__ Cx<R>* fft_any(Cx<R>* a, Cx<R>* b, int nseq, const FftPlanD& pl, const Cx<R>* tw) {
+  switch (pl.n) {
+    case 256: return fft_ct_stage<R, 256, 0, 1, NT, INV>(a, b, nseq, tw);
+    case 320: return fft_ct_stage<R, 320, 0, 1, NT, INV>(a, b, nseq, tw);
+    case 368: return fft_ct_stage<R, 368, 0, 1, NT, INV>(a, b, nseq, tw);
+    case 384: return fft_ct_stage<R, 384, 0, 1, NT, INV>(a, b, nseq, tw);
+    case 512: return fft_ct_stage<R, 512, 0, 1, NT, INV>(a, b, nseq, tw);
+    case 640: return fft_ct_stage<R, 640, 0, 1, NT, INV>(a, b, nseq, tw);
+    case 736: return fft_ct_stage<R, 736, 0, 1, NT, INV>(a, b, nseq, tw);
+    case 768: return fft_ct_stage<R, 768, 0, 1, NT, INV>(a, b, nseq, tw);
+    default: return lds_fft<R, NT>(a, b, nseq, pl, tw, INV);
+  }
+}
+
 // numpy shift maps: ifftshift moves i to (i + ceil(n/2)) % n (the prefft /
 // preifft index map of mricartesianoperator.cu:44-49); fftshift output k
 // reads input (k - n/2) mod n.
-__device__ __forceinline__ int ishift_dst(int i, int n) { return (i + (n + 1) / 2) % n; }
-__device__ __forceinline__ int fshift_src(int k, int n) { return (k + n - n / 2) % n; }
+// Both take 0 <= i < n and avoid the integer modulo.
+__device__ __forceinline__ int ishift_dst(int i, int n) {
+  const int j = i + (n + 1) / 2;
+  return j >= n ? j - n : j;
+}
+__device__ __forceinline__ int fshift_src(int k, int n) {
+  const int j = k + n - n / 2;
+  return j >= n ? j - n : j;
+}
+// ifftshift gather: position j of ifftshift(z) holds z[(j + n/2) mod n].
+__device__ __forceinline__ int ishift_src(int j, int n) {
+  const int i = j + n / 2;
+  return i >= n ? i - n : i;
+}
 
 // ---------------------------------------------------------------------------
 // Stencil helpers on plane pointers (tgv_kernels.cuh:25-135)
@@ -649,13 +745,13 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
   for (int e = threadIdx.x; e < nc; e += NT) L.cols[e] = d.cols[e];
   const C* src = static_cast<const C*>(MODE == M_OP ? d.op_in : d.ub) + size_t(y) * W;
   const C* S = static_cast<const C*>(d.S);
-  const size_t G = size_t(W) * d.Hk;
-  const size_t srow = size_t(y + d.y0) * W;
+  const int G = W * d.Hk;
+  const C* Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : nullptr;  // coil c0, row y + y0
   const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
   for (int e = threadIdx.x; e < ncb * W; e += NT) {
     const int cl = qdiv(e, W, rW), x = e - cl * W;
     C z = src[x];
-    if (S) z = z * S[G * (c0 + cl) + srow + x];
+    if (S) z = z * Sg[cl * G + x];
     L.a[cl * W + ishift_dst(x, W)] = z;
   }
   double t[kGapTerms] = {0};
@@ -667,11 +763,12 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
     for (int x = threadIdx.x; x < W; x += NT) dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, a0, a1, t);
   }
   __syncthreads();
-  C* res = lds_fft<R, NT>(L.a, L.b, ncb, d.planW, L.tw, false);
-  C* T1 = static_cast<C*>(d.T1);
+  C* res = fft_any<R, NT, false>(L.a, L.b, ncb, d.planW, L.tw);
+  C* T1 = static_cast<C*>(d.T1) + (size_t(c0) * H + y) * nc;  // row y of coil c0
+  const int cstride = H * nc;
   for (int e = threadIdx.x; e < ncb * nc; e += NT) {
     const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
-    T1[(size_t(c0 + cl) * H + y) * nc + ks] = res[cl * W + fshift_src(L.cols[ks], W)];
+    T1[cl * cstride + ks] = res[cl * W + fshift_src(L.cols[ks], W)];
   }
   if (GAP) block_sum<3, NT>(reinterpret_cast<double(&)[3]>(t), part + size_t(li) * kGapTerms);
 }
@@ -703,15 +800,15 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   C* res;
   double t[2] = {0, 0};
   if (CMODE != CM_ADJ) {
-    const C* T1 = static_cast<const C*>(d.T1);
+    const C* T1 = static_cast<const C*>(d.T1) + size_t(c) * H * nc + ks0;  // coil c, column ks0
     for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
       const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
-      const int fe = (j + Hk / 2) % Hk;  // ifftshift gather
+      const int fe = ishift_src(j, Hk);
       const int yy = fe - y0;
-      bufA[cl * Hk + j] = (yy >= 0 && yy < H) ? T1[(size_t(c) * H + yy) * nc + ks0 + cl] : pzero<C>();
+      bufA[cl * Hk + j] = (yy >= 0 && yy < H) ? T1[yy * nc + cl] : pzero<C>();
     }
     __syncthreads();
-    res = lds_fft<R, NT>(bufA, bufB, ncg, d.planH, tw, false);
+    res = fft_any<R, NT, false>(bufA, bufB, ncg, d.planH, tw);
     C* other = (res == bufA) ? bufB : bufA;
     if (CMODE == CM_FWD) {
       C* out = static_cast<C*>(d.op_out);
@@ -724,16 +821,17 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
     }
     // TGV_prox_r_kernel (tgv_kernels.cuh:212-223) on A ubar - f
     const R sigma = R(d.sigma), lambda = R(d.lambda);
-    const C* fk = static_cast<const C*>(d.fk);
-    C* rk = static_cast<C*>(d.rk);
+    // columns ks0.. of coil c are contiguous in the compact layout
+    const C* fk = static_cast<const C*>(d.fk) + (size_t(c) * nc + ks0) * Hk;
+    C* rk = static_cast<C*>(d.rk) + (size_t(c) * nc + ks0) * Hk;
+    const R* mk = Mc + size_t(ks0) * Hk;
     for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
       const int cl = qdiv(e, Hk, rHk), kfe = e - cl * Hk;
-      const R m = Mc[size_t(ks0 + cl) * Hk + kfe];
-      const size_t idx = (size_t(c) * nc + ks0 + cl) * Hk + kfe;
+      const R m = mk[e];
       const C a = (m * scale) * res[cl * Hk + fshift_src(kfe, Hk)];
-      const C tmp = a - fk[idx];
-      const C rn = (rk[idx] + sigma * tmp) / (R(1) + sigma * lambda);
-      rk[idx] = rn;
+      const C tmp = a - fk[e];
+      const C rn = (rk[e] + sigma * tmp) / (R(1) + sigma * lambda);
+      rk[e] = rn;
       if (GAP) {
         t[0] += dabs2(tmp);
         t[1] += dabs2(tmp - lambda * rn);
@@ -741,7 +839,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
       other[cl * Hk + ishift_dst(kfe, Hk)] = m * rn;
     }
     __syncthreads();
-    res = lds_fft<R, NT>(other, res, ncg, d.planH, tw, true);
+    res = fft_any<R, NT, true>(other, res, ncg, d.planH, tw);
   } else {
     const C* in = static_cast<const C*>(d.op_in);
     for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
@@ -750,13 +848,12 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
       bufA[cl * Hk + ishift_dst(kfe, Hk)] = m * in[(size_t(c) * Hk + kfe) * W + d.cols[ks0 + cl]];
     }
     __syncthreads();
-    res = lds_fft<R, NT>(bufA, bufB, ncg, d.planH, tw, true);
+    res = fft_any<R, NT, true>(bufA, bufB, ncg, d.planH, tw);
   }
-  C* T2 = static_cast<C*>(d.T2);
+  C* T2 = static_cast<C*>(d.T2) + size_t(c) * H * nc + ks0;
   for (int e = threadIdx.x; e < ncg * H; e += NT) {
     const int yy = qdiv(e, ncg, rncg), cl = e - yy * ncg;
-    const int fe = yy + y0;
-    T2[(size_t(c) * H + yy) * nc + ks0 + cl] = scale * res[cl * Hk + fshift_src(fe, Hk)];
+    T2[yy * nc + cl] = scale * res[cl * Hk + fshift_src(yy + y0, Hk)];
   }
   if (GAP) block_sum<2, NT>(t, part + size_t(li) * kGapTerms);
 }
@@ -781,22 +878,23 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   for (int e = threadIdx.x; e < ncb * W; e += NT) L.a[e] = pzero<C>();
   for (int e = threadIdx.x; e < nc; e += NT) L.cols[e] = d.cols[e];
   __syncthreads();
-  const C* T2 = static_cast<const C*>(d.T2);
+  const C* T2 = static_cast<const C*>(d.T2) + (size_t(c0) * H + y) * nc;  // row y of coil c0
+  const int cstride = H * nc;
   const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
   for (int e = threadIdx.x; e < ncb * nc; e += NT) {
     const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
-    L.a[cl * W + ishift_dst(L.cols[ks], W)] = T2[(size_t(c0 + cl) * H + y) * nc + ks];
+    L.a[cl * W + ishift_dst(L.cols[ks], W)] = T2[cl * cstride + ks];
   }
   __syncthreads();
-  C* res = lds_fft<R, NT>(L.a, L.b, ncb, d.planW, L.tw, true);
+  C* res = fft_any<R, NT, true>(L.a, L.b, ncb, d.planW, L.tw);
   C* other = (res == L.a) ? L.b : L.a;
   const C* S = static_cast<const C*>(d.S);
-  const size_t G = size_t(W) * d.Hk;
-  const size_t srow = size_t(y + d.y0) * W;
+  const int G = W * d.Hk;
+  const C* Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : nullptr;
   for (int e = threadIdx.x; e < ncb * W; e += NT) {
     const int cl = qdiv(e, W, rW), x = e - cl * W;
     const C v = res[cl * W + fshift_src(x, W)];
-    other[cl * W + x] = S ? mulconj(v, S[G * (c0 + cl) + srow + x]) : v;
+    other[cl * W + x] = Sg ? mulconj(v, Sg[cl * G + x]) : v;
   }
   __syncthreads();
   C* wp = static_cast<C*>(d.wpart) + (size_t(g) * H + y) * W;
