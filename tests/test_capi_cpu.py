@@ -1,0 +1,142 @@
+"""C ABI and host logic, CPU only (no kernel is launched).
+
+The library must load here (it links libamdhip64 but needs no GPU for the
+calls below), export every symbol include/pdt_capi.h declares, and raise the
+reference's errors for the host-detectable failures.
+"""
+import ctypes as ct
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pdt_capi.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"PDT_API\s+[\w\s\*]+?\b(pdt_\w+)\s*\(", txt)))
+
+
+def test_header_declares_and_library_exports_every_symbol():
+    from primaldualtoolbox import _capi as C
+    names = _declared()
+    assert len(names) >= 30
+    L = C.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert missing == []
+    out = subprocess.run(["nm", "-D", "--defined-only", C.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (pdt_\w+)", out))
+    assert set(names) == exported
+    assert sorted(C.EXPORTS) == names
+
+
+def test_version_and_errors_without_gpu():
+    from primaldualtoolbox import _capi as C
+    L = C.lib()
+    assert L.pdt_version() >= 100
+    h = ct.c_void_p()
+    assert L.pdt_solver_create(99, C.PDT_F32, ct.byref(h)) == C.PDT_ERR_ARG
+    assert b"unknown solver kind" in L.pdt_last_error()
+    assert L.pdt_op_create(C.PDT_OP_CARTESIAN, C.PDT_F32, ct.byref(h)) == C.PDT_ERR_ARG
+    assert L.pdt_solver_create(C.PDT_SOLVER_TGV_MRI, C.PDT_F32, ct.byref(h)) == C.PDT_ERR_UNSUPPORTED
+
+
+def test_reference_state_errors():
+    """tvoptimizer.cu:79-85, tgvmrioptimizer.cu:96-110, tgvoptimizerwithop.cu:41-47."""
+    import primaldualtoolbox as pdt
+    from primaldualtoolbox import _capi as C
+    opt = pdt.denoising.TvOptimizer_2f()
+    with pytest.raises(C.PdtError, match="No noisy data is set!") as e:
+        opt.solve()
+    assert e.value.code == C.PDT_ERR_STATE
+    mri = pdt.mri.TgvMriOptimizer_2c3c()
+    with pytest.raises(C.PdtError, match="No operator is set!"):
+        mri.solve()
+    with pytest.raises(C.PdtError, match="There is no result yet!"):
+        pdt.mri.TgvOptimizer_2c3c().getResult()
+    with pytest.raises(AttributeError):
+        opt.setOperator(None)
+
+
+def test_parameter_defaults_and_fields():
+    """TvParameters/TgvParameters/TgvMriParameters constructors
+    (tvoptimizer.cu:26-29, tgvoptimizer.cu:27-30, tgvmrioptimizer.cu:28-31)
+    and the Python field names (pyDenoising.cpp:48-61, pyMriReconstruction.cpp:121-143)."""
+    import primaldualtoolbox as pdt
+    tv = pdt.denoising.TvOptimizer_2f().getParameters()
+    assert (tv.Lambda, tv.max_iter, tv.check) == (1.0, 1, 100)
+    tgv = pdt.denoising.TgvOptimizer_2c().getParameters()
+    assert (tgv.Lambda, tgv.alpha0, tgv.alpha1, tgv.max_iter, tgv.check) == (1.0, 1.0, 1.0, 1, 100)
+    m = pdt.mri.TgvMriOptimizer_2c3c().getParameters()
+    assert (m.alpha0, m.alpha1, m.reduction, m.max_iter, m.check) == (1.0, 1.0, 1.0, 1, 100)
+    with pytest.raises(AttributeError):
+        m.Lambda = 3  # TgvMriParameters has no lambda
+    with pytest.raises(AttributeError):
+        tv.alpha0 = 2  # TvParameters has no alpha
+    with pytest.raises(OverflowError):
+        tv.max_iter = -1
+    tv.max_iter = 499
+    assert tv.max_iter == 499
+    m.gap_tol, m.gap_every = 1e-4, 50
+    assert (m.gap_tol, m.gap_every) == (1e-4, 50)
+    assert "max_iter=1" in str(tgv) and "alpha0=1.0" in str(tgv)
+
+
+def test_python_api_surface():
+    """Class and method names of the reference modules."""
+    import primaldualtoolbox as pdt
+    for name in ["TvOptimizer_2f", "TvOptimizer_2c", "TgvOptimizer_2f", "TgvOptimizer_2c", "TvParameters",
+                 "TgvParameters"]:
+        assert hasattr(pdt.denoising, name)
+    for name in ["MriCartesianOperator", "MriCartesianRemoveROOSOperator", "MriSamplingOperator",
+                 "TgvMriOptimizer_2c3c", "TgvOptimizer_2c3c", "TvOptimizerWithOp_2c3c", "TgvMriOptimizer_2c2c",
+                 "TgvOptimizer_2c2c", "TvOptimizer_2c2c", "TgvMriParameters"]:
+        assert hasattr(pdt.mri, name)
+    for m in ["setNoisyData", "solve", "getResult", "getParameters", "setInput0", "setOperator"]:
+        assert callable(getattr(pdt.mri.TgvMriOptimizer_2c3c, m))
+    for m in ["setMask", "setCoilSens", "forward", "adjoint"]:
+        assert callable(getattr(pdt.mri.MriCartesianRemoveROOSOperator, m))
+
+
+def test_numpy_iu_axis_convention():
+    """numpy (C, FE, PE) <-> iu {PE, FE, C} (python/moduleutils.h:29-72)."""
+    from primaldualtoolbox import _capi as C
+    assert C.iu_size((15, 640, 320)) == [320, 640, 15]
+    assert C.iu_size((320, 320)) == [320, 320]
+
+
+def test_synthetic_generators_follow_spec():
+    from primaldualtoolbox import synthetic as syn
+    rng = np.random.default_rng(0)
+    img = syn.ellipse_phantom(rng, 64)
+    assert img.min() >= 0 and img.max() <= 1
+    S = syn.coil_sensitivities(rng, 6, 40, 20)
+    assert np.allclose(np.sum(np.abs(S) ** 2, axis=0), 1.0)
+    m = syn.line_mask(rng, 40, 320, accel=4, center=24)
+    lines = np.flatnonzero(m[0])
+    assert len(lines) == 80 and np.all(m == m[0:1])
+    assert set(range(160 - 12, 160 + 12)) <= set(lines.tolist())
+    case = syn.mri_case(seed=0, n=32, coils=3, center=4)
+    f, mask = case["kspace"], case["mask"]
+    assert np.all(f[:, mask == 0] == 0)  # data only on sampled lines
+
+
+def test_cpp_mirror_compiles():
+    """The header-only C++ mirror of the reference classes compiles against
+    the C ABI (host compiler, no GPU)."""
+    src = os.path.join(ROOT, "tests", "cpp", "mirror_example.cpp")
+    out = os.path.join("/tmp", "pdt_mirror_example")
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", f"-I{os.path.join(ROOT, 'include')}", src, "-o", out,
+           f"-L{os.path.dirname(_lib_path())}", "-lpdt_hip", f"-Wl,-rpath,{os.path.dirname(_lib_path())}",
+           "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def _lib_path():
+    from primaldualtoolbox import _capi as C
+    return C.LIB_PATH
