@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=50)
     ap.add_argument("--fixed-iters", type=int, default=0,
                     help="profiling mode: run this many iterations per step (no gap stop), print no JSON")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the tile cache (PDT_TUNE_CACHE) or defaults instead of timing candidates")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="profiling mode: run a plain device copy of this many MiB (PMC byte calibration)")
     args = ap.parse_args()
@@ -123,7 +125,7 @@ def main():
     C.check(C.lib().pdt_op_adjoint(op._h, f_dev.h, u0_dev.h))
 
     import primaldualtoolbox as pdt
-    tune = pdt.mri.autotune([opt], reps=20)
+    tune = None if args.no_autotune else pdt.mri.autotune([opt], reps=20)
     log(f"[bench] autotuned tiles: {tune}")
     if args.fixed_iters:
         prm = opt.getParameters()

@@ -169,13 +169,12 @@ PDT_API int pdt_solver_get_v(pdt_solver* s, pdt_array** v);
 PDT_API int pdt_solver_get_stats(const pdt_solver* s, pdt_stats* st);
 PDT_API int pdt_solver_destroy(pdt_solver* s);
 
-/* NEW: per-kernel timing of a configured solver.  Each fused kernel of one
- * iteration is launched `reps` times back to back on the solver's stream,
- * bracketed by hipEvents; reported per kernel: the average launch duration
- * and the algorithmic HBM bytes of one launch (every distinct element the
- * launch reads or writes, once, at storage precision).  Starts from the
- * solver's current state and leaves it advanced by arbitrary steps: call it
- * after the solves whose results are used. */
+/* NEW: per-kernel timing of a configured solver.  Runs `reps` iterations on
+ * the solver's stream with a hipEvent pair around every fused kernel launch;
+ * reported per kernel: the average launch duration and the algorithmic HBM
+ * bytes of one launch (every distinct element the launch reads or writes,
+ * once, at storage precision).  Starts from the solver's current state and
+ * leaves it advanced: call it after the solves whose results are used. */
 typedef struct pdt_kernel_prof {
   char name[32];
   double ms;
