@@ -81,6 +81,188 @@ def cpu_baseline(case, a1, gpu_iters: int, sample_iters: int, threads: int):
     }
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    return O
+
+
+def _rel(a, b) -> float:
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _timed_solves(opt, u0_dev, steps: int) -> float:
+    """Median wall time of `steps` solves, each restarted from u0 on the device."""
+    from primaldualtoolbox import _capi as C
+    ts = []
+    for _ in range(steps):
+        C.check(C.lib().pdt_solver_set_input0(opt._h, u0_dev.h, None))
+        t0 = time.perf_counter()
+        opt.solve()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def extra_tv256(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
+    """ROF denoising, 256^2 float32, lambda 10, 500 iterations.  Accuracy is
+    the relative L2 distance to the FP64 GPU solver on the same input."""
+    import primaldualtoolbox as pdt
+    from primaldualtoolbox import _capi as C
+    from primaldualtoolbox import synthetic as syn
+    _, noisy = syn.tv_denoising_case(seed=0, n=256, sigma=0.1)
+    res = {}
+    for name, dt in (("f32", np.float32), ("f64", np.float64)):
+        opt = pdt.denoising.TvOptimizer_2f(dtype=dt)
+        opt.setNoisyData(noisy.astype(dt))
+        p = opt.getParameters()
+        p.Lambda, p.max_iter, p.gap_every = 10, 499, 500
+        f_dev = C.DeviceArray.from_numpy(noisy.astype(dt), dt)
+        opt.solve()
+        res[name] = (opt, _timed_solves(opt, f_dev, steps), opt.getResult().copy())
+    opt, s, u = res["f32"]
+    st = opt.getStats()
+    out = {"workload": "TV-L2 (ROF) 256x256 float32, lambda=10, 500 iterations",
+           "seconds": s, "us_per_iteration": s * 1e6 / 500, "rel_gap": float(st["rel_gap"]),
+           "rel_l2_vs_f64": _rel(u, res["f64"][2]), "seconds_f64": res["f64"][1],
+           "kernels": {k["name"]: round(k["ms"], 5) for k in opt.profileKernels(50)}}
+    if not no_cpu:
+        O = _oracle()
+        O.set_threads(cpu_threads)
+        _, _, ost, _ = O.solve(O.TV, noisy, lambda_=10, max_iter=499, gap_every=500, prec=32)
+        out["cpu_baseline"] = {"value": ost["seconds"], "unit": "s", "cores": cpu_threads, "kind": "port",
+                               "sample": "the full 500 iterations in the FP32 OpenMP oracle"}
+    return out
+
+
+def extra_tgv512(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
+    """TGV^2 denoising, 512^2 complex64, lambda 20, alpha0 2, alpha1 1, 1000
+    iterations, FP32 and FP16 dual storage; accuracy against the FP64 GPU
+    solver on the same input."""
+    import primaldualtoolbox as pdt
+    from primaldualtoolbox import _capi as C
+    from primaldualtoolbox import synthetic as syn
+    _, noisy = syn.tgv_denoising_case(seed=0, n=512, sigma=0.05)
+    res = {}
+    for name, dt, fp16 in (("f32", np.complex64, 0), ("f16dual", np.complex64, 1), ("f64", np.complex128, 0)):
+        opt = pdt.denoising.TgvOptimizer_2c(dtype=dt)
+        opt.setNoisyData(noisy.astype(dt))
+        p = opt.getParameters()
+        p.Lambda, p.alpha0, p.alpha1, p.max_iter, p.dual_fp16, p.gap_every = 20, 2, 1, 999, fp16, 1000
+        f_dev = C.DeviceArray.from_numpy(noisy.astype(dt), dt)
+        opt.solve()
+        res[name] = (opt, _timed_solves(opt, f_dev, steps), opt.getResult().copy())
+    ref = res["f64"][2]
+    out = {"workload": "TGV2-L2 512x512 complex64, lambda=20, alpha0=2, alpha1=1, 1000 iterations"}
+    for name in ("f32", "f16dual"):
+        opt, s, u = res[name]
+        out[name] = {"seconds": s, "us_per_iteration": s * 1e6 / 1000, "rel_l2_vs_f64": _rel(u, ref),
+                     "rel_gap": float(opt.getStats()["rel_gap"]),
+                     "kernels": {k["name"]: round(k["ms"], 5) for k in opt.profileKernels(50)}}
+    out["seconds_f64"] = res["f64"][1]
+    if not no_cpu:
+        O = _oracle()
+        O.set_threads(cpu_threads)
+        n_it = 100
+        _, _, ost, _ = O.solve(O.TGV, noisy, lambda_=20, alpha0=2, alpha1=1, max_iter=n_it - 1,
+                               gap_every=n_it, prec=32)
+        out["cpu_baseline"] = {"value": ost["seconds"] * 1000 / n_it, "unit": "s", "cores": cpu_threads,
+                               "kind": "port",
+                               "sample": f"{n_it} iterations in the FP32 OpenMP oracle, scaled to 1000"}
+    return out
+
+
+def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool) -> dict:
+    """`count` mixed Cartesian TGV-MRI slices (sizes 256^2..384x320, 8-20
+    coils, random or regular masks) solved together to rel. gap 1e-4 in one
+    batched solve; reports slices/s and per-slice iterations."""
+    import primaldualtoolbox as pdt
+    from primaldualtoolbox import _capi as C
+    from primaldualtoolbox import synthetic as syn
+    t0 = time.perf_counter()
+    cases = syn.batch_cases(range(1000, 1000 + count), workers=workers)
+    log(f"[bench] batch: generated {count} slices in {time.perf_counter() - t0:.1f} s")
+    opts, u0s, a1s = [], [], []
+    for case in cases:
+        op = pdt.mri.MriCartesianRemoveROOSOperator()
+        op.setCoilSens(case["sens"])
+        op.setMask(case["mask"])
+        opt = pdt.mri.TgvMriOptimizer_2c3c()
+        opt.setOperator(op)
+        opt.setNoisyData(case["kspace"])
+        a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
+        prm = opt.getParameters()
+        prm.alpha1, prm.alpha0, prm.reduction, prm.max_iter = a1, 2 * a1, 1.0, 200000
+        prm.gap_tol, prm.gap_every = GAP_TOL, GAP_EVERY
+        f_dev = C.DeviceArray.from_numpy(case["kspace"], np.complex64)
+        H, W = case["image"].shape
+        u0 = C.DeviceArray.empty(C.PDT_C64, [W, H])
+        C.check(C.lib().pdt_op_adjoint(op._h, f_dev.h, u0.h))
+        opts.append((op, opt))
+        u0s.append(u0)
+        a1s.append(a1)
+    solvers = [o for _, o in opts]
+    tune = pdt.mri.autotune(solvers, reps=5)
+    log(f"[bench] batch: tiles {tune}")
+
+    def cold():
+        for o, u0, case in zip(solvers, u0s, cases):
+            o.setNoisyData(case["kspace"])
+            C.check(C.lib().pdt_solver_set_input0(o._h, u0.h, None))
+
+    cold()
+    pdt.mri.batch_solve(solvers)  # warmup
+    cold()
+    t0 = time.perf_counter()
+    pdt.mri.batch_solve(solvers)
+    batch_s = time.perf_counter() - t0
+    stats = [o.getStats() for o in solvers]
+    its = [int(s["iters"]) for s in stats]
+    if not all(s["converged"] for s in stats):
+        raise RuntimeError("batched solve: a slice did not reach the gap")
+    log(f"[bench] batch: {count} slices in {batch_s:.3f} s, iterations {min(its)}..{max(its)}")
+    # the same slices one at a time (single-slice engine per solver)
+    cold()
+    t0 = time.perf_counter()
+    for o in solvers:
+        o.solve()
+    seq_s = time.perf_counter() - t0
+    log(f"[bench] batch: sequential {seq_s:.3f} s")
+    work = [int(s["iters"]) * c["meta"]["coils"] * 2 * c["meta"]["n"] * c["meta"]["n_pe"]
+            for s, c in zip(stats, cases)]
+    out = {"workload": (f"{count} Cartesian TGV-MRI slices (seeds 1000..{999 + count}), sizes "
+                        f"{sorted(set((c['meta']['n'], c['meta']['n_pe']) for c in cases))}, 8-20 coils, "
+                        "rel. gap 1e-4 per slice, cold start"),
+           "seconds": batch_s, "slices_per_s": count / batch_s,
+           "sequential_seconds": seq_s, "sequential_slices_per_s": count / seq_s,
+           "iterations": {"min": min(its), "median": int(np.median(its)), "max": max(its)},
+           "tiles": tune}
+    if not no_cpu:
+        O = _oracle()
+        import concurrent.futures as cf
+        n_s, n_it = min(cpu_threads, count), 10
+
+        def one(i):
+            O.set_threads(1)
+            c = cases[i]
+            oop = O.Op(O.OP_CARTESIAN_ROOS, c["sens"], c["mask"], prec=32)
+            _, _, ost, _ = O.solve(O.TGV_MRI, c["kspace"], op=oop, alpha1=a1s[i], alpha0=2 * a1s[i],
+                                   max_iter=n_it - 1, prec=32)
+            return ost["seconds"], n_it * c["meta"]["coils"] * 2 * c["meta"]["n"] * c["meta"]["n_pe"]
+
+        with cf.ThreadPoolExecutor(max_workers=n_s) as ex:
+            sample = list(ex.map(one, range(n_s)))
+        per_unit = sum(t for t, _ in sample) / sum(u for _, u in sample)
+        cpu_s = per_unit * sum(work) / n_s
+        out["cpu_baseline"] = {"value": count / cpu_s, "unit": "slices/s", "cores": n_s, "kind": "port",
+                               "sample": (f"{n_it} iterations of each of the first {n_s} slices in the FP32 "
+                                          f"oracle, one slice per thread; cost per coil-pixel-iteration "
+                                          f"scaled to the {count} slices' GPU iteration counts and divided "
+                                          f"over {n_s} threads")}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +278,10 @@ def main():
                     help="use the tile cache (PDT_TUNE_CACHE) or defaults instead of timing candidates")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="profiling mode: run a plain device copy of this many MiB (PMC byte calibration)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the TV 256^2, TGV 512^2 and batched-slices side measurements")
+    ap.add_argument("--batch", type=int, default=64, help="slices in the batched side measurement")
+    ap.add_argument("--gen-workers", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -185,6 +371,20 @@ def main():
         log(f"[bench] CPU baseline: {args.cpu_iters} iterations on {args.cpu_threads} threads")
         cpu = cpu_baseline(case, a1, int(np.median(iters)), args.cpu_iters, args.cpu_threads)
 
+    extra = None
+    if not args.no_extra:
+        extra = {}
+        for key, fn in (("tv_256", lambda: extra_tv256(3, args.cpu_threads, args.no_cpu)),
+                        ("tgv_512", lambda: extra_tgv512(3, args.cpu_threads, args.no_cpu)),
+                        (f"batch_{args.batch}", lambda: extra_batch64(args.batch, args.gen_workers,
+                                                                      args.cpu_threads, args.no_cpu))):
+            log(f"[bench] side measurement {key}")
+            try:
+                extra[key] = fn()
+            except Exception as e:  # reported, never hides the headline line
+                extra[key] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] {key}: {extra[key]}")
+
     line = {
         "metric": "time-to-gap 1e-4 (s), 15-coil 320^2 Cartesian TGV, cold start",
         "value": value,
@@ -220,6 +420,7 @@ def main():
             "traffic": traffic,
         },
         "cpu_baseline": cpu,
+        "extra": extra,
     }
     print(json.dumps(line), flush=True)
 

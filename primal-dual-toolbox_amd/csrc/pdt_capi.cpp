@@ -474,6 +474,8 @@ struct pdt_op {
     L.cg = cg;
     L.lds_rows = lds_rows;
     L.lds_cols = lds_cols;
+    L.fft_w = W;
+    L.fft_h = Hk;
     const int dt = dtype == PDT_C128 ? DT_C128 : DT_C64;
     if (forward) {
       run_zero(st, dst, size_t(W) * Hk * C * esize(cplx_of(dtype)));
@@ -673,7 +675,10 @@ struct Engine {
   int* h_ctl = nullptr;       // pinned mirrors
   double* h_stats = nullptr;
   int nrows = 0, nrg = 0, ncolitems = 0;
-  Launch L;
+  Launch L;                             // every slice: gap reduction, full-size buffers
+  std::vector<Launch> G;                // one launch per group of slices with equal FFT lengths
+  std::vector<std::vector<int>> groups;  // slice indices of each group
+  int listed_items = 0;                 // items in the current work lists
   std::vector<SliceDev> hs;
   struct GraphEntry {
     int len;
@@ -725,9 +730,8 @@ struct Engine {
       TuneVal tv = tune_lookup(key);
       rpi = force_rpi > 0 ? force_rpi : tv.rpi > 0 ? tv.rpi : 1;
     }
-    // slices and work lists
+    // slices, grouped by FFT lengths, and their work lists
     hs.assign(ns, SliceDev{});
-    std::vector<int2> rows, rgs, cols;
     for (int i = 0; i < ns; ++i) {
       pdt_solver* s = sv[i];
       SliceDev d = mri ? s->op->geometry() : SliceDev{};
@@ -789,22 +793,30 @@ struct Engine {
         s->wpart.ensure(size_t(d.ngr) * d.W * d.H * s->pe());
         d.wpart = s->wpart.p;
       }
-      d.row_beg = (int)rows.size();
-      const int step = mri ? 1 : rpi;
-      for (int y = 0; y < s->H; y += step) rows.push_back(make_int2(i, y));
-      d.row_end = (int)rows.size();
-      d.rg_beg = (int)rgs.size();
-      if (mri)
-        for (int y = 0; y < s->H; ++y)
-          for (int g = 0; g < d.ngr; ++g) rgs.push_back(make_int2(i, y * d.ngr + g));
-      d.rg_end = (int)rgs.size();
-      d.col_beg = (int)cols.size();
-      if (mri)
-        for (int c = 0; c < d.C; ++c)
-          for (int g = 0; g < d.ngroups; ++g) cols.push_back(make_int2(i, c * d.ngroups + g));
-      d.col_end = (int)cols.size();
       hs[i] = d;
     }
+    groups.clear();
+    if (mri) {
+      std::map<std::pair<int, int>, size_t> gi;
+      for (int i = 0; i < ns; ++i) {
+        const auto k = std::make_pair(hs[i].W, hs[i].Hk);
+        auto it = gi.find(k);
+        if (it == gi.end()) {
+          it = gi.emplace(k, groups.size()).first;
+          groups.emplace_back();
+        }
+        groups[it->second].push_back(i);
+      }
+    } else {
+      groups.emplace_back();
+      for (int i = 0; i < ns; ++i) groups[0].push_back(i);
+    }
+    L.rpi = rpi;
+    L.cb = cb;
+    L.cg = cg;
+    std::vector<int2> rows, rgs, cols;
+    std::vector<GroupRange> gr;
+    make_lists(std::vector<char>(ns, 1), rows, rgs, cols, gr);
     nrows = (int)rows.size();
     nrg = (int)rgs.size();
     ncolitems = (int)cols.size();
@@ -813,12 +825,7 @@ struct Engine {
     sig.assign(reinterpret_cast<const char*>(hs.data()), hs.size() * sizeof(SliceDev));
     const int tiles[4] = {cb, cg, rpi, int(sv[0]->prm.gap_every)};
     sig.append(reinterpret_cast<const char*>(tiles), sizeof(tiles));
-    if (plan_only) {
-      L.cb = cb;
-      L.cg = cg;
-      L.rpi = rpi;
-      return;
-    }
+    if (plan_only) return;
     d_ctl.alloc(size_t(ns) * CTL_WORDS * sizeof(int));
     d_stats.alloc(size_t(ns) * ST_WORDS * sizeof(double));
     HIPC(hipHostMalloc((void**)&h_ctl, size_t(ns) * CTL_WORDS * sizeof(int), hipHostMallocDefault));
@@ -828,13 +835,9 @@ struct Engine {
       hs[i].stats = d_stats.as<double>() + size_t(i) * ST_WORDS;
     }
     d_slices.alloc(hs.size() * sizeof(SliceDev));
-    HIPC(hipMemcpy(d_slices.p, hs.data(), hs.size() * sizeof(SliceDev), hipMemcpyHostToDevice));
     d_rows.alloc(std::max<size_t>(1, rows.size()) * sizeof(int2));
-    HIPC(hipMemcpy(d_rows.p, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
     d_rg.alloc(std::max<size_t>(1, rgs.size()) * sizeof(int2));
-    if (!rgs.empty()) HIPC(hipMemcpy(d_rg.p, rgs.data(), rgs.size() * sizeof(int2), hipMemcpyHostToDevice));
     d_cols.alloc(std::max<size_t>(1, cols.size()) * sizeof(int2));
-    if (!cols.empty()) HIPC(hipMemcpy(d_cols.p, cols.data(), cols.size() * sizeof(int2), hipMemcpyHostToDevice));
     const size_t nitem = std::max<size_t>(1, std::max({rows.size(), rgs.size(), cols.size()}));
     d_part.alloc(4 * nitem * kGapTerms * sizeof(double));
     L.stream = stream;
@@ -844,14 +847,125 @@ struct Engine {
     L.nrows = nrows;
     L.rowgroups = d_rg.as<int2>();
     L.nrg = nrg;
-    L.rpi = rpi;
     L.cols = d_cols.as<int2>();
     L.ncolitems = ncolitems;
     for (int k = 0; k < 4; ++k) L.part[k] = d_part.as<double>() + size_t(k) * nitem * kGapTerms;
-    L.cb = cb;
-    L.cg = cg;
     L.lds_rows = lr;
     L.lds_cols = lc;
+    upload_lists(rows, rgs, cols, gr);
+  }
+
+  // Item ranges of one slice group in the work lists.
+  struct GroupRange {
+    int g;  // index into groups
+    int row0, nrow, rg0, nrg, col0, ncol;
+  };
+
+  // Work lists of the slices with use[i] != 0, group after group: row items
+  // (slice, first row), row-group items (slice, row * ngr + coil group) and
+  // column items (slice, coil * ngroups + column group).  Sets the item
+  // ranges of hs; slices left out get empty ranges.
+  void make_lists(const std::vector<char>& use, std::vector<int2>& rows, std::vector<int2>& rgs,
+                  std::vector<int2>& cols, std::vector<GroupRange>& gr) {
+    const bool mri = sv[0]->is_op();
+    const int step = mri ? 1 : L.rpi;
+    for (size_t g = 0; g < groups.size(); ++g) {
+      GroupRange r{int(g), (int)rows.size(), 0, (int)rgs.size(), 0, (int)cols.size(), 0};
+      for (int i : groups[g]) {
+        SliceDev& d = hs[i];
+        d.row_beg = d.row_end = d.rg_beg = d.rg_end = d.col_beg = d.col_end = 0;
+        if (!use[i]) continue;
+        d.row_beg = (int)rows.size();
+        for (int y = 0; y < d.H; y += step) rows.push_back(make_int2(i, y));
+        d.row_end = (int)rows.size();
+        d.rg_beg = (int)rgs.size();
+        if (mri)
+          for (int y = 0; y < d.H; ++y)
+            for (int q = 0; q < d.ngr; ++q) rgs.push_back(make_int2(i, y * d.ngr + q));
+        d.rg_end = (int)rgs.size();
+        d.col_beg = (int)cols.size();
+        if (mri)
+          for (int c = 0; c < d.C; ++c)
+            for (int q = 0; q < d.ngroups; ++q) cols.push_back(make_int2(i, c * d.ngroups + q));
+        d.col_end = (int)cols.size();
+      }
+      r.nrow = (int)rows.size() - r.row0;
+      r.nrg = (int)rgs.size() - r.rg0;
+      r.ncol = (int)cols.size() - r.col0;
+      if (r.nrow > 0) gr.push_back(r);
+    }
+  }
+
+  // Copies slices and lists to the device (the stream is idle) and derives
+  // one launch per non-empty group: its sub-ranges of the lists and of the
+  // partial-sum buffers, its FFT lengths and LDS sizes.
+  void upload_lists(const std::vector<int2>& rows, const std::vector<int2>& rgs, const std::vector<int2>& cols,
+                    const std::vector<GroupRange>& gr) {
+    HIPC(hipMemcpy(d_slices.p, hs.data(), hs.size() * sizeof(SliceDev), hipMemcpyHostToDevice));
+    if (!rows.empty()) HIPC(hipMemcpy(d_rows.p, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (!rgs.empty()) HIPC(hipMemcpy(d_rg.p, rgs.data(), rgs.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (!cols.empty()) HIPC(hipMemcpy(d_cols.p, cols.data(), cols.size() * sizeof(int2), hipMemcpyHostToDevice));
+    const bool mri = sv[0]->is_op();
+    const int dt = dtk();
+    G.clear();
+    listed_items = 0;
+    for (const GroupRange& r : gr) {
+      Launch x = L;
+      x.rows = L.rows + r.row0;
+      x.nrows = r.nrow;
+      x.rowgroups = L.rowgroups + r.rg0;
+      x.nrg = r.nrg;
+      x.cols = L.cols + r.col0;
+      x.ncolitems = r.ncol;
+      if (mri) {
+        // rows_fwd writes per row-group item, cols per column item, primal per row item
+        x.part[0] = L.part[0] + size_t(r.rg0) * kGapTerms;
+        x.part[1] = L.part[1] + size_t(r.col0) * kGapTerms;
+        x.part[2] = L.part[2] + size_t(r.row0) * kGapTerms;
+        const SliceDev& d0 = hs[groups[r.g][0]];
+        int nc = 1;
+        for (int i : groups[r.g]) nc = std::max(nc, hs[i].ncols);
+        x.fft_w = d0.W;
+        x.fft_h = d0.Hk;
+        x.lds_rows = lds_rows_bytes(dt, x.cb, d0.W, nc);
+        x.lds_cols = lds_cols_bytes(dt, x.cg, d0.Hk);
+      }
+      listed_items += r.nrow + r.nrg + r.ncol;
+      G.push_back(x);
+    }
+  }
+
+  // Drops the items of finished slices from the work lists once they make
+  // up more than a sixth of them (batched solves whose slices stop at
+  // different iterations); the captured graphs are rebuilt on next use.
+  void compact_lists() {
+    const int ns = (int)sv.size();
+    if (ns < 2) return;
+    std::vector<char> use(ns, 0);
+    int items = 0;
+    for (int i = 0; i < ns; ++i) {
+      use[i] = h_ctl[size_t(i) * CTL_WORDS + CTL_ACTIVE] != 0;
+      if (use[i]) {
+        const SliceDev& d = hs[i];
+        const int step = sv[0]->is_op() ? 1 : L.rpi;
+        items += (d.H + step - 1) / step + (sv[0]->is_op() ? d.H * d.ngr + d.C * d.ngroups : 0);
+      }
+    }
+    if (items == 0 || 6 * items > 5 * listed_items) return;
+    std::vector<int2> rows, rgs, cols;
+    std::vector<GroupRange> gr;
+    make_lists(use, rows, rgs, cols, gr);
+    upload_lists(rows, rgs, cols, gr);
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
+
+  // One stage (fused kernel) of an iteration over every group.
+  void stage(int st, bool gap) {
+    for (const Launch& x : G) run_stage(x, kind, dtk(), fp16, gap, st);
+  }
+  void iteration(bool gap) {
+    for (int st = 0; st < num_stages(kind); ++st) stage(st, gap);
   }
 
   // Algorithmic HBM bytes of one launch of a stage, summed over the slices.
@@ -904,7 +1018,7 @@ struct Engine {
     const int dt = dtk();
     for (int i = 0; i < len; ++i) {
       const bool g = gap && i == len - 1;
-      run_iteration(L, kind, dt, fp16, g);
+      iteration(g);
       if (g) run_gap_reduce(L, kind, dt);
     }
     const int ns = (int)sv.size();
@@ -985,6 +1099,7 @@ struct Engine {
       if (!any) break;
       if (changed)
         HIPC(hipMemcpyAsync(d_ctl.p, h_ctl, size_t(ns) * CTL_WORDS * sizeof(int), hipMemcpyHostToDevice, stream));
+      compact_lists();
     }
     HIPC(hipEventRecord(e1, stream));
     HIPC(hipEventSynchronize(e1));
@@ -1463,15 +1578,14 @@ int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, int cap, i
     // every stage timed inside real iterations (stage order as in a solve),
     // an event pair around each launch
     const int ns = num_stages(e.kind);
-    const int dt = e.dtk();
     *n = std::min(ns, cap);
-    run_iteration(e.L, e.kind, dt, e.fp16, false);  // warm the code objects
+    e.iteration(false);  // warm the code objects
     std::vector<hipEvent_t> ev(size_t(2) * ns * reps);
     for (auto& x : ev) HIPC(hipEventCreate(&x));
     for (int r = 0; r < reps; ++r)
       for (int st = 0; st < ns; ++st) {
         HIPC(hipEventRecord(ev[(size_t(r) * ns + st) * 2], e.stream));
-        run_stage(e.L, e.kind, dt, e.fp16, false, st);
+        e.stage(st, false);
         HIPC(hipEventRecord(ev[(size_t(r) * ns + st) * 2 + 1], e.stream));
       }
     HIPC(hipStreamSynchronize(e.stream));
@@ -1498,12 +1612,11 @@ double time_iteration(Engine& e, int reps) {
   hipEvent_t e0, e1;
   HIPC(hipEventCreate(&e0));
   HIPC(hipEventCreate(&e1));
-  const int dt = e.dtk();
   double tot = 0;
   for (int st = 0; st < num_stages(e.kind); ++st) {
-    run_stage(e.L, e.kind, dt, e.fp16, false, st);
+    e.stage(st, false);
     HIPC(hipEventRecord(e0, e.stream));
-    for (int r = 0; r < reps; ++r) run_stage(e.L, e.kind, dt, e.fp16, false, st);
+    for (int r = 0; r < reps; ++r) e.stage(st, false);
     HIPC(hipEventRecord(e1, e.stream));
     HIPC(hipEventSynchronize(e1));
     HIPC(hipGetLastError());

@@ -28,6 +28,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "pdt_desc.hpp"
 #include "pdt_launch.hpp"
@@ -231,18 +232,65 @@ __device__ __forceinline__ void butterfly(const Cx<R>* in, Cx<R>* out, int nr, i
   for (int q = 0; q < RR; ++q) out[q * p] = x[q];
 }
 
-// Generic radix (odd primes such as 23): direct DFT reading from LDS.
+// Generic radix (primes without a specialisation): direct DFT reading from
+// LDS; w^(j q mod r) is stepped instead of reduced with a modulo.
 template <class R>
 __device__ void butterfly_gen(const Cx<R>* in, Cx<R>* out, int r, int nr, int p, const Cx<R>* tw,
                               int ts, bool inv) {
   for (int q = 0; q < r; ++q) {
-    Cx<R> acc = pzero<Cx<R>>();
-    for (int j = 0; j < r; ++j) {
-      Cx<R> v = in[j * nr];
-      if (j > 0) v = v * twd(tw, j * ts, inv);
-      acc = acc + v * twd(tw, ((j * q) % r) * nr, inv);
+    Cx<R> acc = in[0];
+    int jq = 0;
+    for (int j = 1; j < r; ++j) {
+      jq += q;
+      if (jq >= r) jq -= r;
+      const Cx<R> v = in[j * nr] * twd(tw, j * ts, inv);
+      acc = acc + v * twd(tw, jq * nr, inv);
     }
     out[q * p] = acc;
+  }
+}
+
+// Odd radix P (7, 11, 13, 23, ...) in the symmetric form: with
+// s_m = x_m + x_(P-m), d_m = x_m - x_(P-m), m = 1..h, h = (P-1)/2,
+//   X_0 = x_0 + sum_m s_m,
+//   X_k, X_(P-k) = x_0 + sum_m cos(2 pi m k / P) s_m  +/-  i sg sum_m sin(2 pi m k / P) d_m,
+// sg = -1 forward, +1 inverse.  The coefficients come from the length-n
+// twiddle table, tw[(m k mod P) nr] = (cos, -sin)(2 pi m k / P), and the
+// loop over output pairs stays rolled: only x_0, s and d are live across it,
+// which keeps this rarely used radix from raising the register count of the
+// kernels that contain it.  h^2 complex-by-real products per output pair.
+template <class R, int P>
+__device__ __forceinline__ void butterfly_odd(const Cx<R>* in, Cx<R>* out, int nr, int p, const Cx<R>* tw,
+                                              int ts, bool inv) {
+  constexpr int h = (P - 1) / 2;
+  const Cx<R> x0 = in[0];
+  Cx<R> s[h], d[h];
+  Cx<R> sum = x0;
+#pragma unroll
+  for (int m = 1; m <= h; ++m) {
+    const Cx<R> a = in[m * nr] * twd(tw, m * ts, inv);
+    const Cx<R> b = in[(P - m) * nr] * twd(tw, (P - m) * ts, inv);
+    s[m - 1] = a + b;
+    d[m - 1] = a - b;
+    sum = sum + s[m - 1];
+  }
+  out[0] = sum;
+  const R sg = inv ? R(1) : R(-1);
+#pragma unroll 1
+  for (int k = 1; k <= h; ++k) {
+    Cx<R> A = x0, B = pzero<Cx<R>>();
+    int jk = 0;
+#pragma unroll
+    for (int m = 1; m <= h; ++m) {
+      jk += k;
+      if (jk >= P) jk -= P;
+      const Cx<R> w = tw[jk * nr];
+      A = A + s[m - 1] * w.re;
+      B = B - d[m - 1] * w.im;
+    }
+    const Cx<R> iB = muli(B, sg);
+    out[k * p] = A + iB;
+    out[(P - k) * p] = A - iB;
   }
 }
 
@@ -252,6 +300,12 @@ __device__ __forceinline__ int qdiv(int a, int b, float rb) {
   if (q * b > a) --q;
   else if ((q + 1) * b <= a) ++q;
   return q;
+}
+// a / n where n == NF when NF > 0 (a division by a constant), else qdiv.
+template <int NF>
+__device__ __forceinline__ int divn(int a, int n, float rn) {
+  if constexpr (NF > 0) return a / NF;
+  else return qdiv(a, n, rn);
 }
 
 // Transforms nseq sequences of length pl.n stored back to back in a; b is
@@ -334,9 +388,8 @@ __device__ __forceinline__ void butterfly_ct(const Cx<R>* in, Cx<R>* out, int ts
 #pragma unroll
     for (int q = 0; q < RR; ++q) out[q * p] = x[q];
   } else {
-    // odd prime radix: direct DFT reading its inputs from LDS (keeps the
-    // register footprint of the kernel small)
-    butterfly_gen<R>(in, out, RR, nr, p, tw, ts, inv);
+    static_assert(RR % 2 == 1, "even radices are factored into 2, 4 and 8");
+    butterfly_odd<R, RR>(in, out, nr, p, tw, ts, inv);
   }
 }
 
@@ -358,20 +411,14 @@ __device__ __forceinline__ Cx<R>* fft_ct_stage(Cx<R>* a, Cx<R>* b, int nseq, con
   }
 }
 
-// Lengths of the benchmark workloads: PE 256/320/368/384, FE 512/640/736/768.
-template <class R, int NT, bool INV>
-__device__ Cx<R>* fft_any(Cx<R>* a, Cx<R>* b, int nseq, const FftPlanD& pl, const Cx<R>* tw) {
-  switch (pl.n) {
-    case 256: return fft_ct_stage<R, 256, 0, 1, NT, INV>(a, b, nseq, tw);
-    case 320: return fft_ct_stage<R, 320, 0, 1, NT, INV>(a, b, nseq, tw);
-    case 368: return fft_ct_stage<R, 368, 0, 1, NT, INV>(a, b, nseq, tw);
-    case 384: return fft_ct_stage<R, 384, 0, 1, NT, INV>(a, b, nseq, tw);
-    case 512: return fft_ct_stage<R, 512, 0, 1, NT, INV>(a, b, nseq, tw);
-    case 640: return fft_ct_stage<R, 640, 0, 1, NT, INV>(a, b, nseq, tw);
-    case 736: return fft_ct_stage<R, 736, 0, 1, NT, INV>(a, b, nseq, tw);
-    case 768: return fft_ct_stage<R, 768, 0, 1, NT, INV>(a, b, nseq, tw);
-    default: return lds_fft<R, NT>(a, b, nseq, pl, tw, INV);
-  }
+// FFT of length NF with a compile-time plan, or of length pl.n with the
+// runtime plan when NF == 0.  Each FFT kernel is instantiated per length, so
+// its register budget is that of its own plan only (the runtime plan with
+// its radix switch needs several times the registers of a fixed one).
+template <class R, int NT, bool INV, int NF>
+__device__ __forceinline__ Cx<R>* fft_n(Cx<R>* a, Cx<R>* b, int nseq, const FftPlanD& pl, const Cx<R>* tw) {
+  if constexpr (NF > 0) return fft_ct_stage<R, NF, 0, 1, NT, INV>(a, b, nseq, tw);
+  else return lds_fft<R, NT>(a, b, nseq, pl, tw, INV);
 }
 
 // numpy shift maps: ifftshift moves i to (i + ceil(n/2)) % n (the prefft /
@@ -727,7 +774,7 @@ struct RowLds {
 // y.  Group 0 also performs the dual step of row y (TV or TGV).  Then for
 // each coil of the group: S_c * src row -> ifftshift -> PE-FFT -> fftshift
 // -> sampled columns -> T1 [c][y][ks].
-template <class R, int MODE, bool GAP, int NT>
+template <class R, int MODE, bool GAP, int NT, int NF>
 __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2* items, int nitems, int cb,
                                                  double* part) {
   extern __shared__ char smem_raw[];
@@ -736,7 +783,7 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using C = Cx<R>;
-  const int W = d.W, H = d.H, nc = d.ncols;
+  const int W = NF > 0 ? NF : d.W, H = d.H, nc = d.ncols;
   const int y = it.y / d.ngr, g = it.y - y * d.ngr;
   const int c0 = g * cb, ncb = min(cb, d.C - c0);
   RowLds<C> L(smem_raw, W, cb);
@@ -749,7 +796,7 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
   const C* Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : nullptr;  // coil c0, row y + y0
   const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
   for (int e = threadIdx.x; e < ncb * W; e += NT) {
-    const int cl = qdiv(e, W, rW), x = e - cl * W;
+    const int cl = divn<NF>(e, W, rW), x = e - cl * W;
     C z = src[x];
     if (S) z = z * Sg[cl * G + x];
     L.a[cl * W + ishift_dst(x, W)] = z;
@@ -763,7 +810,7 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
     for (int x = threadIdx.x; x < W; x += NT) dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, a0, a1, t);
   }
   __syncthreads();
-  C* res = fft_any<R, NT, false>(L.a, L.b, ncb, d.planW, L.tw);
+  C* res = fft_n<R, NT, false, NF>(L.a, L.b, ncb, d.planW, L.tw);
   C* T1 = static_cast<C*>(d.T1) + (size_t(c0) * H + y) * nc;  // row y of coil c0
   const int cstride = H * nc;
   for (int e = threadIdx.x; e < ncb * nc; e += NT) {
@@ -777,7 +824,7 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
 // forward (writes op_out {W, Hk, C}), 2 operator adjoint (reads op_in).
 enum { CM_SOLVE = 0, CM_FWD = 1, CM_ADJ = 2 };
 
-template <class R, int CMODE, bool GAP, int NT>
+template <class R, int CMODE, bool GAP, int NT, int NF>
 __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* items, int nitems, int cg,
                                              double* part) {
   extern __shared__ char smem_raw[];
@@ -786,7 +833,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using C = Cx<R>;
-  const int W = d.W, H = d.H, Hk = d.Hk, y0 = d.y0, nc = d.ncols;
+  const int W = d.W, H = d.H, Hk = NF > 0 ? NF : d.Hk, y0 = d.y0, nc = d.ncols;
   const int c = it.y / d.ngroups, g = it.y - c * d.ngroups;
   const int ks0 = g * cg, ncg = min(cg, nc - ks0);
   const R scale = R(1) / sqrt(R(W) * R(Hk));
@@ -808,12 +855,12 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
       bufA[cl * Hk + j] = (yy >= 0 && yy < H) ? T1[yy * nc + cl] : pzero<C>();
     }
     __syncthreads();
-    res = fft_any<R, NT, false>(bufA, bufB, ncg, d.planH, tw);
+    res = fft_n<R, NT, false, NF>(bufA, bufB, ncg, d.planH, tw);
     C* other = (res == bufA) ? bufB : bufA;
     if (CMODE == CM_FWD) {
       C* out = static_cast<C*>(d.op_out);
       for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
-        const int cl = qdiv(e, Hk, rHk), kfe = e - cl * Hk;
+        const int cl = divn<NF>(e, Hk, rHk), kfe = e - cl * Hk;
         const R m = Mc[size_t(ks0 + cl) * Hk + kfe];
         out[(size_t(c) * Hk + kfe) * W + d.cols[ks0 + cl]] = (m * scale) * res[cl * Hk + fshift_src(kfe, Hk)];
       }
@@ -826,7 +873,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
     C* rk = static_cast<C*>(d.rk) + (size_t(c) * nc + ks0) * Hk;
     const R* mk = Mc + size_t(ks0) * Hk;
     for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
-      const int cl = qdiv(e, Hk, rHk), kfe = e - cl * Hk;
+      const int cl = divn<NF>(e, Hk, rHk), kfe = e - cl * Hk;
       const R m = mk[e];
       const C a = (m * scale) * res[cl * Hk + fshift_src(kfe, Hk)];
       const C tmp = a - fk[e];
@@ -839,16 +886,16 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
       other[cl * Hk + ishift_dst(kfe, Hk)] = m * rn;
     }
     __syncthreads();
-    res = fft_any<R, NT, true>(other, res, ncg, d.planH, tw);
+    res = fft_n<R, NT, true, NF>(other, res, ncg, d.planH, tw);
   } else {
     const C* in = static_cast<const C*>(d.op_in);
     for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
-      const int cl = qdiv(e, Hk, rHk), kfe = e - cl * Hk;
+      const int cl = divn<NF>(e, Hk, rHk), kfe = e - cl * Hk;
       const R m = Mc[size_t(ks0 + cl) * Hk + kfe];
       bufA[cl * Hk + ishift_dst(kfe, Hk)] = m * in[(size_t(c) * Hk + kfe) * W + d.cols[ks0 + cl]];
     }
     __syncthreads();
-    res = fft_any<R, NT, true>(bufA, bufB, ncg, d.planH, tw);
+    res = fft_n<R, NT, true, NF>(bufA, bufB, ncg, d.planH, tw);
   }
   C* T2 = static_cast<C*>(d.T2) + size_t(c) * H * nc + ks0;
   for (int e = threadIdx.x; e < ncg * H; e += NT) {
@@ -861,7 +908,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
 // Row pass, adjoint.  Item = (slice, y * ngr + g): for each coil of group g,
 // T2 row -> ifftshift -> inverse PE-FFT -> fftshift -> * conj(S_c); the
 // group's coils are summed in order into wpart[g][y][x].
-template <class R, int NT>
+template <class R, int NT, int NF>
 __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2* items, int nitems, int cb) {
   extern __shared__ char smem_raw[];
   const int li = xcd_item(blockIdx.x, nitems);
@@ -869,7 +916,7 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using C = Cx<R>;
-  const int W = d.W, H = d.H, nc = d.ncols;
+  const int W = NF > 0 ? NF : d.W, H = d.H, nc = d.ncols;
   const int y = it.y / d.ngr, g = it.y - y * d.ngr;
   const int c0 = g * cb, ncb = min(cb, d.C - c0);
   RowLds<C> L(smem_raw, W, cb);
@@ -886,13 +933,13 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
     L.a[cl * W + ishift_dst(L.cols[ks], W)] = T2[cl * cstride + ks];
   }
   __syncthreads();
-  C* res = fft_any<R, NT, true>(L.a, L.b, ncb, d.planW, L.tw);
+  C* res = fft_n<R, NT, true, NF>(L.a, L.b, ncb, d.planW, L.tw);
   C* other = (res == L.a) ? L.b : L.a;
   const C* S = static_cast<const C*>(d.S);
   const int G = W * d.Hk;
   const C* Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : nullptr;
   for (int e = threadIdx.x; e < ncb * W; e += NT) {
-    const int cl = qdiv(e, W, rW), x = e - cl * W;
+    const int cl = divn<NF>(e, W, rW), x = e - cl * W;
     const C v = res[cl * W + fshift_src(x, W)];
     other[cl * W + x] = Sg ? mulconj(v, Sg[cl * G + x]) : v;
   }
@@ -1146,19 +1193,53 @@ static void denoise_stage(const Launch& L, int kind, bool gap, int stage) {
 #define PDT_LAUNCH_NT(kern, grid, nt, lds, st, ...) \
   hipLaunchKernelGGL(kern, dim3(grid), dim3(nt), lds, st, __VA_ARGS__)
 
-// MRI stages: 0 rows_fwd, 1 cols, 2 rows_adj, 3 primal.
+// FFT lengths with kernels built on a compile-time plan (single precision);
+// any other length, and every double-precision launch, uses the runtime plan.
+constexpr int kFftLengths[] = {256, 320, 368, 384, 512, 640, 736, 768};
+
+// Calls fn(std::integral_constant<int, NF>) with NF = n when n has a
+// compile-time plan for R, else NF = 0.
+template <class R, class Fn>
+static void with_fft_len(int n, Fn&& fn) {
+  if constexpr (std::is_same<R, float>::value) {
+    switch (n) {
+      case 256: fn(std::integral_constant<int, 256>{}); return;
+      case 320: fn(std::integral_constant<int, 320>{}); return;
+      case 368: fn(std::integral_constant<int, 368>{}); return;
+      case 384: fn(std::integral_constant<int, 384>{}); return;
+      case 512: fn(std::integral_constant<int, 512>{}); return;
+      case 640: fn(std::integral_constant<int, 640>{}); return;
+      case 736: fn(std::integral_constant<int, 736>{}); return;
+      case 768: fn(std::integral_constant<int, 768>{}); return;
+      default: break;
+    }
+  }
+  fn(std::integral_constant<int, 0>{});
+}
+
+// MRI stages: 0 rows_fwd, 1 cols, 2 rows_adj, 3 primal.  The FFT kernels are
+// picked by L.fft_w (row length) and L.fft_h (column length).
 template <class R, int MODE>
 static void mri_stage_m(const Launch& L, bool gap, int stage) {
   constexpr int NR = kRowThreads, NC = kColThreads;
   if (stage == 0) {
-    if (gap) PDT_LAUNCH_NT((k_rows_fwd<R, MODE, true, NR>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0]);
-    else PDT_LAUNCH_NT((k_rows_fwd<R, MODE, false, NR>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0]);
+    with_fft_len<R>(L.fft_w, [&](auto nf) {
+      constexpr int NF = decltype(nf)::value;
+      if (gap) PDT_LAUNCH_NT((k_rows_fwd<R, MODE, true, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0]);
+      else PDT_LAUNCH_NT((k_rows_fwd<R, MODE, false, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0]);
+    });
   } else if (stage == 1) {
     if (L.ncolitems == 0) return;
-    if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
-    else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+    with_fft_len<R>(L.fft_h, [&](auto nf) {
+      constexpr int NF = decltype(nf)::value;
+      if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+      else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+    });
   } else if (stage == 2) {
-    PDT_LAUNCH_NT((k_rows_adj<R, NR>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb);
+    with_fft_len<R>(L.fft_w, [&](auto nf) {
+      constexpr int NF = decltype(nf)::value;
+      PDT_LAUNCH_NT((k_rows_adj<R, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb);
+    });
   } else {
     if (gap) PDT_LAUNCH((k_mri_primal<R, MODE, true>), L.nrows, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
     else PDT_LAUNCH((k_mri_primal<R, MODE, false>), L.nrows, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
@@ -1205,29 +1286,54 @@ void run_gap_reduce(const Launch& L, int kind, int dtype) {
 template <class R>
 static void op_forward(const Launch& L) {
   constexpr int NR = kRowThreads, NC = kColThreads;
-  PDT_LAUNCH_NT((k_rows_fwd<R, M_OP, false, NR>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0]);
-  PDT_LAUNCH_NT((k_cols<R, CM_FWD, false, NC>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+  with_fft_len<R>(L.fft_w, [&](auto nf) {
+    constexpr int NF = decltype(nf)::value;
+    PDT_LAUNCH_NT((k_rows_fwd<R, M_OP, false, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0]);
+  });
+  with_fft_len<R>(L.fft_h, [&](auto nf) {
+    constexpr int NF = decltype(nf)::value;
+    PDT_LAUNCH_NT((k_cols<R, CM_FWD, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+  });
 }
 
 template <class R>
 static void op_adjoint(const Launch& L) {
   constexpr int NR = kRowThreads, NC = kColThreads;
-  PDT_LAUNCH_NT((k_cols<R, CM_ADJ, false, NC>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
-  PDT_LAUNCH_NT((k_rows_adj<R, NR>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb);
+  with_fft_len<R>(L.fft_h, [&](auto nf) {
+    constexpr int NF = decltype(nf)::value;
+    PDT_LAUNCH_NT((k_cols<R, CM_ADJ, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+  });
+  with_fft_len<R>(L.fft_w, [&](auto nf) {
+    constexpr int NF = decltype(nf)::value;
+    PDT_LAUNCH_NT((k_rows_adj<R, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb);
+  });
   PDT_LAUNCH((k_mri_primal<R, M_OP, false>), L.nrows, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
 }
 
+// Raises the dynamic LDS limit of every FFT kernel instantiation to b bytes.
 template <class R>
 static bool set_lds_attrs(int b) {
   constexpr int NR = kRowThreads, NC = kColThreads;
-  const void* fns[] = {
-      (const void*)&k_rows_fwd<R, M_OP, false, NR>, (const void*)&k_rows_fwd<R, M_TV, false, NR>,
-      (const void*)&k_rows_fwd<R, M_TV, true, NR>,  (const void*)&k_rows_fwd<R, M_TGV, false, NR>,
-      (const void*)&k_rows_fwd<R, M_TGV, true, NR>, (const void*)&k_rows_adj<R, NR>,
-      (const void*)&k_cols<R, CM_SOLVE, false, NC>, (const void*)&k_cols<R, CM_SOLVE, true, NC>,
-      (const void*)&k_cols<R, CM_FWD, false, NC>,   (const void*)&k_cols<R, CM_ADJ, false, NC>};
   bool ok = true;
-  for (const void* f : fns) ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, b) == hipSuccess;
+  auto set = [&](const void* f) {
+    ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, b) == hipSuccess;
+  };
+  auto one = [&](auto nf) {
+    constexpr int NF = decltype(nf)::value;
+    set((const void*)&k_rows_fwd<R, M_OP, false, NR, NF>);
+    set((const void*)&k_rows_fwd<R, M_TV, false, NR, NF>);
+    set((const void*)&k_rows_fwd<R, M_TV, true, NR, NF>);
+    set((const void*)&k_rows_fwd<R, M_TGV, false, NR, NF>);
+    set((const void*)&k_rows_fwd<R, M_TGV, true, NR, NF>);
+    set((const void*)&k_rows_adj<R, NR, NF>);
+    set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF>);
+    set((const void*)&k_cols<R, CM_SOLVE, true, NC, NF>);
+    set((const void*)&k_cols<R, CM_FWD, false, NC, NF>);
+    set((const void*)&k_cols<R, CM_ADJ, false, NC, NF>);
+  };
+  with_fft_len<R>(0, one);
+  for (int n : kFftLengths)
+    if (std::is_same<R, float>::value) with_fft_len<R>(n, one);
   return ok;
 }
 

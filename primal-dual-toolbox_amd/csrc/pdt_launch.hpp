@@ -26,6 +26,8 @@ struct Launch {
   int cg = 1;                    // columns per group in the column kernel
   size_t lds_rows = 0;           // dynamic LDS bytes of the row kernels
   size_t lds_cols = 0;           // dynamic LDS bytes of the column kernel
+  int fft_w = 0;                 // row FFT length shared by every slice of the launch (0: mixed)
+  int fft_h = 0;                 // column FFT length shared by every slice (0: mixed)
 };
 
 // One full iteration of the solver kind on every active slice.

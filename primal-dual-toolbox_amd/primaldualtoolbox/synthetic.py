@@ -74,7 +74,8 @@ def coil_sensitivities(rng: np.random.Generator, coils: int, n_fe: int, n_pe: in
     0.6 N, times a smooth random first-order phase, normalised so that
     sum_c |S_c|^2 = 1 at every pixel.  Shape ``(coils, n_fe, n_pe)``."""
     size = float(size if size is not None else min(n_fe, n_pe))
-    yy, xx = np.mgrid[0:n_fe, 0:n_pe].astype(np.float64)
+    ys = np.arange(n_fe, dtype=np.float64)
+    xs = np.arange(n_pe, dtype=np.float64)
     cy, cx = (n_fe - 1) / 2.0, (n_pe - 1) / 2.0
     out = np.empty((coils, n_fe, n_pe), np.complex128)
     w = 0.4 * size
@@ -82,12 +83,14 @@ def coil_sensitivities(rng: np.random.Generator, coils: int, n_fe: int, n_pe: in
         ang = 2.0 * np.pi * c / coils + rng.uniform(-0.2, 0.2)
         px = cx + 0.6 * size * np.cos(ang)
         py = cy + 0.6 * size * np.sin(ang)
-        mag = np.exp(-((xx - px) ** 2 + (yy - py) ** 2) / (2.0 * w * w))
         k = rng.uniform(-1, 1, size=3)
-        phase = k[0] * np.pi + k[1] * np.pi * (xx - cx) / n_pe + k[2] * np.pi * (yy - cy) / n_fe
-        out[c] = mag * np.exp(1j * phase)
-    norm = np.sqrt(np.sum(np.abs(out) ** 2, axis=0))
-    return out / norm[None]
+        # bump and phase are separable in (y, x): an outer product of 1D factors
+        fy = np.exp(-((ys - py) ** 2) / (2.0 * w * w) + 1j * (k[0] * np.pi + k[2] * np.pi * (ys - cy) / n_fe))
+        fx = np.exp(-((xs - px) ** 2) / (2.0 * w * w) + 1j * (k[1] * np.pi * (xs - cx) / n_pe))
+        out[c] = np.outer(fy, fx)
+    norm2 = np.sum(out.real ** 2 + out.imag ** 2, axis=0)
+    out *= (1.0 / np.sqrt(norm2))[None]
+    return out
 
 
 def line_mask(rng: np.random.Generator, n_fe: int, n_pe: int, accel: float = 4.0,
@@ -118,9 +121,15 @@ def roos_offset(n_fe_k: int) -> int:
 
 
 def centered_fft2(z: np.ndarray, inverse: bool = False) -> np.ndarray:
+    """fftshift(FFT2(ifftshift(z))) orthonormal over the last two axes."""
     ax = (-2, -1)
-    f = np.fft.ifft2 if inverse else np.fft.fft2
-    return np.fft.fftshift(f(np.fft.ifftshift(z, axes=ax), axes=ax, norm="ortho"), axes=ax)
+    try:
+        import scipy.fft as sfft
+        f = sfft.ifft2 if inverse else sfft.fft2
+        return sfft.fftshift(f(sfft.ifftshift(z, axes=ax), axes=ax, norm="ortho", workers=-1), axes=ax)
+    except ImportError:
+        f = np.fft.ifft2 if inverse else np.fft.fft2
+        return np.fft.fftshift(f(np.fft.ifftshift(z, axes=ax), axes=ax, norm="ortho"), axes=ax)
 
 
 def cartesian_forward_np(x: np.ndarray, sens: np.ndarray, mask: np.ndarray,
@@ -165,3 +174,30 @@ def mri_case(seed: int = 0, n: int = 320, coils: int = 15, n_pe: int | None = No
         "kspace": f.astype(np.complex64),
         "roos": roos,
     }
+
+
+# Matrix sizes (FE x PE of the image) and ranges of the batched workload.
+BATCH_SIZES = ((256, 256), (320, 320), (368, 368), (384, 320))
+
+
+def batch_case(seed: int):
+    """One slice of the batched workload: size from BATCH_SIZES, coils
+    U{8..20}, random or regular 4x mask with 16..32 centre lines."""
+    rng = np.random.default_rng(seed)
+    n, n_pe = BATCH_SIZES[int(rng.integers(0, len(BATCH_SIZES)))]
+    coils = int(rng.integers(8, 21))
+    center = int(rng.integers(16, 33))
+    regular = bool(rng.integers(0, 2))
+    case = mri_case(seed=seed, n=n, coils=coils, n_pe=n_pe, center=center, regular=regular)
+    case["meta"] = {"seed": seed, "n": n, "n_pe": n_pe, "coils": coils, "center": center, "regular": regular}
+    return case
+
+
+def batch_cases(seeds, workers: int = 16):
+    """Generates batch_case(seed) for each seed in a process pool."""
+    import concurrent.futures as cf
+    seeds = list(seeds)
+    if workers <= 1 or len(seeds) <= 1:
+        return [batch_case(s) for s in seeds]
+    with cf.ProcessPoolExecutor(max_workers=min(workers, len(seeds))) as ex:
+        return list(ex.map(batch_case, seeds))

@@ -102,6 +102,70 @@ def test_cfg3_full_size_parity_fixed_iterations(oracle):
     assert err < 1e-3
 
 
+@pytest.mark.parametrize("n,n_pe", [(256, 256), (368, 368), (384, 320), (160, 384)])
+def test_fft_lengths_solver_parity(oracle, n, n_pe):
+    """TGV-MRI through the per-length FFT kernels: PE 256/368/320/384 and FE
+    512/736/768/320 (320 = 2n for n = 160), 4 coils, 10 iterations vs the
+    FP64 oracle."""
+    pdt = _pdt()
+    from primaldualtoolbox import synthetic as syn
+    case = syn.mri_case(seed=3, n=n, n_pe=n_pe, coils=4)
+    a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
+    print(f"fft lengths PE {n_pe} FE {2 * n}", flush=True)
+    _, opt = _mri_solver(pdt, case, a1=a1, max_iter=9)
+    opt.solve()
+    oop = oracle.Op(oracle.OP_CARTESIAN_ROOS, case["sens"].astype(np.complex128), case["mask"].astype(np.float64))
+    ref, _, _, _ = oracle.solve(oracle.TGV_MRI, case["kspace"].astype(np.complex128), op=oop, alpha1=a1,
+                                alpha0=2 * a1, max_iter=9)
+    err = rel_l2(opt.getResult(), ref)
+    print(f"  rel L2 {err:.3e}", flush=True)
+    assert err < 1e-3
+
+
+def _batch_pair(pdt, specs):
+    """Two identical lists of configured solvers (for single vs batched)."""
+    from primaldualtoolbox import synthetic as syn
+    out = ([], [])
+    for n, n_pe, coils, seed, max_iter, tol in specs:
+        case = syn.mri_case(seed=seed, n=n, n_pe=n_pe, coils=coils)
+        a1 = 1e-4 * float(np.linalg.norm(case["kspace"]))
+        for store in out:
+            store.append(_mri_solver(pdt, case, a1=a1, max_iter=max_iter, gap_every=50, gap_tol=tol))
+    return out
+
+
+def test_batch_ragged_stops_equal_single_solves():
+    """Slices of one batch that stop at different iterations (iteration
+    limits and gap tolerances) leave the work lists as they finish; every
+    slice's result, iteration count and gap equal those of its own solve."""
+    pdt = _pdt()
+    specs = [(32, 32, 3, 1, 49, 0.0), (24, 24, 2, 2, 299, 0.0), (32, 32, 4, 3, 999, 1e-2),
+             (20, 28, 2, 4, 149, 0.0), (32, 32, 3, 5, 999, 3e-3), (24, 24, 2, 6, 99, 0.0)]
+    singles, batch = _batch_pair(pdt, specs)
+    for _, o in singles:
+        o.solve()
+    pdt.mri.batch_solve([o for _, o in batch])
+    for (_, a), (_, b) in zip(singles, batch):
+        sa, sb = a.getStats(), b.getStats()
+        print(f"  iters single {sa['iters']} batch {sb['iters']}", flush=True)
+        assert sa["iters"] == sb["iters"]
+        assert sa["rel_gap"] == sb["rel_gap"]
+        assert np.array_equal(a.getResult(), b.getResult())
+
+
+def test_batch_mixed_compiled_and_runtime_lengths():
+    """One batch holding slices with compile-time FFT plans (256 / 512) and
+    runtime plans (PE 40, FE 80) equals the single solves bit for bit."""
+    pdt = _pdt()
+    specs = [(256, 256, 2, 7, 29, 0.0), (40, 40, 3, 8, 29, 0.0), (256, 256, 3, 9, 29, 0.0)]
+    singles, batch = _batch_pair(pdt, specs)
+    for _, o in singles:
+        o.solve()
+    pdt.mri.batch_solve([o for _, o in batch])
+    for (_, a), (_, b) in zip(singles, batch):
+        assert np.array_equal(a.getResult(), b.getResult())
+
+
 def test_cfg3_time_to_gap_fp32_matches_fp64_gpu():
     """Size-independent check at full size: the FP32 and FP64 GPU solvers stop
     within one gap interval of each other at rel. gap 1e-4 and agree on u."""
