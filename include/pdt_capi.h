@@ -173,8 +173,11 @@ PDT_API int pdt_solver_destroy(pdt_solver* s);
  * the solver's stream with a hipEvent pair around every fused kernel launch;
  * reported per kernel: the average launch duration and the algorithmic HBM
  * bytes of one launch (every distinct element the launch reads or writes,
- * once, at storage precision).  Starts from the solver's current state and
- * leaves it advanced: call it after the solves whose results are used. */
+ * once, at storage precision).  Stages that launch nothing for this solver
+ * (the separate primal kernel when every slice has one coil group) are
+ * omitted; *n receives the number of entries written.  Starts from the
+ * solver's current state and leaves it advanced: call it after the solves
+ * whose results are used. */
 typedef struct pdt_kernel_prof {
   char name[32];
   double ms;

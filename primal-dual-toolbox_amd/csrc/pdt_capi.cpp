@@ -476,6 +476,8 @@ struct pdt_op {
     L.lds_cols = lds_cols;
     L.fft_w = W;
     L.fft_h = Hk;
+    L.split_primal = d.ngr > 1;
+    L.primal_threads = primal_block(W);
     const int dt = dtype == PDT_C128 ? DT_C128 : DT_C64;
     if (forward) {
       run_zero(st, dst, size_t(W) * Hk * C * esize(cplx_of(dtype)));
@@ -850,6 +852,7 @@ struct Engine {
     L.cols = d_cols.as<int2>();
     L.ncolitems = ncolitems;
     for (int k = 0; k < 4; ++k) L.part[k] = d_part.as<double>() + size_t(k) * nitem * kGapTerms;
+    L.gap_rows = L.part[2];
     L.lds_rows = lr;
     L.lds_cols = lc;
     upload_lists(rows, rgs, cols, gr);
@@ -927,6 +930,9 @@ struct Engine {
         for (int i : groups[r.g]) nc = std::max(nc, hs[i].ncols);
         x.fft_w = d0.W;
         x.fft_h = d0.Hk;
+        x.primal_threads = primal_block(d0.W);
+        x.split_primal = false;
+        for (int i : groups[r.g]) x.split_primal = x.split_primal || hs[i].ngr > 1;
         x.lds_rows = lds_rows_bytes(dt, x.cb, d0.W, nc);
         x.lds_cols = lds_cols_bytes(dt, x.cg, d0.Hk);
       }
@@ -985,10 +991,11 @@ struct Engine {
         const double T = C * H * nc * pe;                                   // T1 or T2
         const bool tgv = kind != PDT_SOLVER_TV_OP;
         const double Wp = double(d.ngr) * N * pe;  // per-coil-group partial adjoints
+        const double Pr = (tgv ? 14 * pe : 5 * pe) * N;  // primal step
         if (stage == 0) tot += (tgv ? 13 * pe : 5 * pe) * N + S + T;
         else if (stage == 1) tot += 2 * T + 3 * C * nc * Hk * pe + nc * Hk * (pe / 2);
-        else if (stage == 2) tot += T + S + Wp;
-        else tot += Wp + (tgv ? 14 * pe : 5 * pe) * N;
+        else if (stage == 2) tot += T + S + (d.ngr > 1 ? Wp : Pr);  // one coil group: primal step in the same kernel
+        else tot += d.ngr > 1 ? Wp + Pr : 0.0;
       }
     }
     return tot;
@@ -1578,7 +1585,6 @@ int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, int cap, i
     // every stage timed inside real iterations (stage order as in a solve),
     // an event pair around each launch
     const int ns = num_stages(e.kind);
-    *n = std::min(ns, cap);
     e.iteration(false);  // warm the code objects
     std::vector<hipEvent_t> ev(size_t(2) * ns * reps);
     for (auto& x : ev) HIPC(hipEventCreate(&x));
@@ -1590,17 +1596,22 @@ int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, int cap, i
       }
     HIPC(hipStreamSynchronize(e.stream));
     HIPC(hipGetLastError());
-    for (int st = 0; st < *n; ++st) {
+    int k = 0;
+    for (int st = 0; st < ns && k < cap; ++st) {
+      const double bytes = e.stage_bytes(st);
+      if (bytes <= 0) continue;  // a stage without launches (primal step done inside k_rows_adj)
       double tot = 0;
       for (int r = 0; r < reps; ++r) {
         float ms = 0;
         HIPC(hipEventElapsedTime(&ms, ev[(size_t(r) * ns + st) * 2], ev[(size_t(r) * ns + st) * 2 + 1]));
         tot += ms;
       }
-      std::snprintf(out[st].name, sizeof(out[st].name), "%s", stage_name(e.kind, st));
-      out[st].ms = tot / reps;
-      out[st].bytes = e.stage_bytes(st);
+      std::snprintf(out[k].name, sizeof(out[k].name), "%s", stage_name(e.kind, st));
+      out[k].ms = tot / reps;
+      out[k].bytes = bytes;
+      ++k;
     }
+    *n = k;
     for (auto& x : ev) (void)hipEventDestroy(x);
   });
 }

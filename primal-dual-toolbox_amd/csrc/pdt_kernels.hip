@@ -127,6 +127,26 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double* out) {
   }
 }
 
+// block_sum for a block of blockDim.x threads (a multiple of 64, at most 1024).
+template <int N>
+__device__ __forceinline__ void block_sum_dyn(double (&v)[N], double* out) {
+  __shared__ double red[16 * kGapTerms];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int t = 0; t < N; ++t) {
+    double x = v[t];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    if (lane == 0) red[wave * kGapTerms + t] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double s = 0;
+    for (int w = 0; w < nw; ++w) s += red[w * kGapTerms + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
 template <int NT = kBlock>
 __device__ __forceinline__ double block_max(double v) {
   __shared__ double red[NT / 64];
@@ -795,11 +815,28 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
   const int G = W * d.Hk;
   const C* Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : nullptr;  // coil c0, row y + y0
   const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
-  for (int e = threadIdx.x; e < ncb * W; e += NT) {
-    const int cl = divn<NF>(e, W, rW), x = e - cl * W;
-    C z = src[x];
-    if (S) z = z * Sg[cl * G + x];
-    L.a[cl * W + ishift_dst(x, W)] = z;
+  // With a compile-time length and at most kPrefetch elements per thread the
+  // row and sensitivity values are loaded into registers first and combined
+  // after the dual step, so that both sets of loads are in flight together.
+  C ps[kPrefetch], pS[kPrefetch];
+  const bool pf = NF > 0 && ncb * W <= kPrefetch * NT;
+  if (pf) {
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < ncb * W) {
+        const int cl = divn<NF>(e, W, rW), x = e - cl * W;
+        ps[k] = src[x];
+        if (S) pS[k] = Sg[cl * G + x];
+      }
+    }
+  } else {
+    for (int e = threadIdx.x; e < ncb * W; e += NT) {
+      const int cl = divn<NF>(e, W, rW), x = e - cl * W;
+      C z = src[x];
+      if (S) z = z * Sg[cl * G + x];
+      L.a[cl * W + ishift_dst(x, W)] = z;
+    }
   }
   double t[kGapTerms] = {0};
   if (MODE != M_OP && g == 0) {
@@ -808,6 +845,16 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
     const R a0 = sched_alpha<R>(d, d.alpha0, d.alpha0_end, k);
     const R a1 = sched_alpha<R>(d, d.alpha1, d.alpha1_end, k);
     for (int x = threadIdx.x; x < W; x += NT) dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, a0, a1, t);
+  }
+  if (pf) {
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < ncb * W) {
+        const int cl = divn<NF>(e, W, rW), x = e - cl * W;
+        L.a[cl * W + ishift_dst(x, W)] = S ? ps[k] * pS[k] : ps[k];
+      }
+    }
   }
   __syncthreads();
   C* res = fft_n<R, NT, false, NF>(L.a, L.b, ncb, d.planW, L.tw);
@@ -846,6 +893,29 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   const float rHk = 1.0f / float(Hk), rncg = 1.0f / float(ncg);
   C* res;
   double t[2] = {0, 0};
+  // TGV_prox_r_kernel (tgv_kernels.cuh:212-223) operands; columns ks0.. of
+  // coil c are contiguous in the compact layout.  With a compile-time length
+  // they are loaded into registers here, together with the T1 gather, so that
+  // their latency is not paid again after the forward FFT.
+  const R sigma = R(d.sigma), lambda = R(d.lambda);
+  const C* fk = static_cast<const C*>(d.fk) + (size_t(c) * nc + ks0) * Hk;
+  C* rk = static_cast<C*>(d.rk) + (size_t(c) * nc + ks0) * Hk;
+  const R* mk = Mc + size_t(ks0) * Hk;
+  constexpr int KP = NF > 0 ? (kColPrefetchCols * NF + NT - 1) / NT : 1;
+  C pf_f[KP], pf_r[KP];
+  R pf_m[KP];
+  const bool pf = NF > 0 && CMODE == CM_SOLVE && ncg <= kColPrefetchCols;
+  if (pf) {
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < ncg * Hk) {
+        pf_f[k] = fk[e];
+        pf_r[k] = rk[e];
+        pf_m[k] = mk[e];
+      }
+    }
+  }
   if (CMODE != CM_ADJ) {
     const C* T1 = static_cast<const C*>(d.T1) + size_t(c) * H * nc + ks0;  // coil c, column ks0
     for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
@@ -866,24 +936,27 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
       }
       return;
     }
-    // TGV_prox_r_kernel (tgv_kernels.cuh:212-223) on A ubar - f
-    const R sigma = R(d.sigma), lambda = R(d.lambda);
-    // columns ks0.. of coil c are contiguous in the compact layout
-    const C* fk = static_cast<const C*>(d.fk) + (size_t(c) * nc + ks0) * Hk;
-    C* rk = static_cast<C*>(d.rk) + (size_t(c) * nc + ks0) * Hk;
-    const R* mk = Mc + size_t(ks0) * Hk;
-    for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
+    // prox of r on A ubar - f
+    auto prox = [&](int e, R m, C f, C r) {
       const int cl = divn<NF>(e, Hk, rHk), kfe = e - cl * Hk;
-      const R m = mk[e];
       const C a = (m * scale) * res[cl * Hk + fshift_src(kfe, Hk)];
-      const C tmp = a - fk[e];
-      const C rn = (rk[e] + sigma * tmp) / (R(1) + sigma * lambda);
+      const C tmp = a - f;
+      const C rn = (r + sigma * tmp) / (R(1) + sigma * lambda);
       rk[e] = rn;
       if (GAP) {
         t[0] += dabs2(tmp);
         t[1] += dabs2(tmp - lambda * rn);
       }
       other[cl * Hk + ishift_dst(kfe, Hk)] = m * rn;
+    };
+    if (pf) {
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int e = threadIdx.x + k * NT;
+        if (e < ncg * Hk) prox(e, pf_m[k], pf_f[k], pf_r[k]);
+      }
+    } else {
+      for (int e = threadIdx.x; e < ncg * Hk; e += NT) prox(e, mk[e], fk[e], rk[e]);
     }
     __syncthreads();
     res = fft_n<R, NT, true, NF>(other, res, ncg, d.planH, tw);
@@ -905,11 +978,41 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   if (GAP) block_sum<2, NT>(t, part + size_t(li) * kGapTerms);
 }
 
+// Primal step of pixel (x, y) given w = (A^* r)(x, y):
+// TGV_primal_u_noprox_kernel (tgv_kernels.cuh:432-467) and, for TGV,
+// TGV_primal_v_kernel (:647-686).  Adds |w - div p|^2 and |p + div_sym q|^2
+// to t for the gap.
+template <class R, int MODE, bool GAP>
+__device__ __forceinline__ void mri_primal_pixel(const SliceDev& d, int x, int y, Cx<R> w, R tau, R theta,
+                                                 double* t) {
+  using C = Cx<R>;
+  const int W = d.W, H = d.H;
+  const size_t n = size_t(W) * H, i = size_t(y) * W + x;
+  C* u = static_cast<C*>(d.u);
+  C* ub = static_cast<C*>(d.ub);
+  const C* px = static_cast<const C*>(d.p);
+  const C* py = px + n;
+  const C uo = u[i];
+  const C div = dmx<C>(px, x, i, W) + dmy<C>(py, y, i, W, H);
+  const C un = uo + tau * (div - w);
+  u[i] = un;
+  ub[i] = (R(1) + theta) * un - theta * uo;
+  if (GAP) t[0] += dabs2(w - div);
+  if (MODE == M_TGV) {
+    const double rv = vstep_pixel<C, C>(d, x, y, tau, theta);
+    if (GAP) t[1] += rv;
+  }
+}
+
 // Row pass, adjoint.  Item = (slice, y * ngr + g): for each coil of group g,
-// T2 row -> ifftshift -> inverse PE-FFT -> fftshift -> * conj(S_c); the
-// group's coils are summed in order into wpart[g][y][x].
-template <class R, int NT, int NF>
-__global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2* items, int nitems, int cb) {
+// T2 row -> ifftshift -> inverse PE-FFT -> fftshift -> * conj(S_c), summed
+// over the group's coils in order.  With several coil groups (ngr > 1) the
+// sum goes to wpart[g][y][x] and k_mri_primal completes the step; with one
+// group the block holds all of (A^* r)(., y) and performs the primal step of
+// row y itself (MODE M_OP: writes A^* r to op_out).
+template <class R, int MODE, bool GAP, int NT, int NF>
+__global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2* items, int nitems, int cb,
+                                                 double* part_rows) {
   extern __shared__ char smem_raw[];
   const int li = xcd_item(blockIdx.x, nitems);
   const int2 it = items[li];
@@ -925,9 +1028,26 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   for (int e = threadIdx.x; e < ncb * W; e += NT) L.a[e] = pzero<C>();
   for (int e = threadIdx.x; e < nc; e += NT) L.cols[e] = d.cols[e];
   __syncthreads();
+  const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
+  // sensitivities of the group's coils on row y + y0: with a compile-time
+  // length loaded into registers before the T2 gather (one latency for both)
+  const C* S = static_cast<const C*>(d.S);
+  const int G = W * d.Hk;
+  const C* Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : nullptr;
+  C pS[kPrefetch];
+  const bool pf = NF > 0 && Sg && ncb * W <= kPrefetch * NT;
+  if (pf) {
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < ncb * W) {
+        const int cl = divn<NF>(e, W, rW), x = e - cl * W;
+        pS[k] = Sg[cl * G + x];
+      }
+    }
+  }
   const C* T2 = static_cast<const C*>(d.T2) + (size_t(c0) * H + y) * nc;  // row y of coil c0
   const int cstride = H * nc;
-  const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
   for (int e = threadIdx.x; e < ncb * nc; e += NT) {
     const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
     L.a[cl * W + ishift_dst(L.cols[ks], W)] = T2[cl * cstride + ks];
@@ -935,69 +1055,80 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   __syncthreads();
   C* res = fft_n<R, NT, true, NF>(L.a, L.b, ncb, d.planW, L.tw);
   C* other = (res == L.a) ? L.b : L.a;
-  const C* S = static_cast<const C*>(d.S);
-  const int G = W * d.Hk;
-  const C* Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : nullptr;
-  for (int e = threadIdx.x; e < ncb * W; e += NT) {
-    const int cl = divn<NF>(e, W, rW), x = e - cl * W;
-    const C v = res[cl * W + fshift_src(x, W)];
-    other[cl * W + x] = Sg ? mulconj(v, Sg[cl * G + x]) : v;
+  if (pf) {
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < ncb * W) {
+        const int cl = divn<NF>(e, W, rW), x = e - cl * W;
+        other[cl * W + x] = mulconj(res[cl * W + fshift_src(x, W)], pS[k]);
+      }
+    }
+  } else {
+    for (int e = threadIdx.x; e < ncb * W; e += NT) {
+      const int cl = divn<NF>(e, W, rW), x = e - cl * W;
+      const C v = res[cl * W + fshift_src(x, W)];
+      other[cl * W + x] = Sg ? mulconj(v, Sg[cl * G + x]) : v;
+    }
   }
   __syncthreads();
-  C* wp = static_cast<C*>(d.wpart) + (size_t(g) * H + y) * W;
+  if (d.ngr > 1) {
+    C* wp = static_cast<C*>(d.wpart) + (size_t(g) * H + y) * W;
+    for (int x = threadIdx.x; x < W; x += NT) {
+      C acc = other[x];
+      for (int cl = 1; cl < ncb; ++cl) acc = acc + other[cl * W + x];
+      wp[x] = acc;
+    }
+    return;
+  }
+  const R tau = R(d.tau), theta = R(d.theta);
+  double t[2] = {0, 0};
   for (int x = threadIdx.x; x < W; x += NT) {
-    C acc = other[x];
-    for (int cl = 1; cl < ncb; ++cl) acc = acc + other[cl * W + x];
-    wp[x] = acc;
+    C w = other[x];
+    for (int cl = 1; cl < ncb; ++cl) w = w + other[cl * W + x];
+    if (MODE == M_OP) static_cast<C*>(d.op_out)[size_t(y) * W + x] = w;
+    else mri_primal_pixel<R, MODE, GAP>(d, x, y, w, tau, theta, t);
+  }
+  if (MODE == M_OP) return;
+  if (GAP) block_sum<2, NT>(t, part_rows + size_t(d.row_beg + y) * kGapTerms);
+  if (y == 0 && threadIdx.x == 0) {
+    d.ctl[CTL_ITER] += 1;
+    d.ctl[CTL_RUPD] += 1;
   }
 }
 
-// Primal half of the MRI solvers: w = sum_g wpart[g] (A^* r), then
-// TGV_primal_u_noprox_kernel (tgv_kernels.cuh:432-467) and, for TGV,
-// TGV_primal_v_kernel (:647-686).  Item = (slice, first row), rpi rows.
-// MODE M_OP writes w to op_out instead (operator adjoint).
+// Primal half of the MRI solvers for slices with several coil groups:
+// w = sum_g wpart[g] (A^* r), then the primal step of each pixel.
+// Item = (slice, first row), rpi rows; the block is sized to cover them in
+// one pass (all loads of a thread issued together).  MODE M_OP writes w to
+// op_out instead (operator adjoint).  Slices with one coil group are done in
+// k_rows_adj.
 template <class R, int MODE, bool GAP>
-__global__ __launch_bounds__(kBlock) void k_mri_primal(const SliceDev* sl, const int2* items, int nitems, int rpi,
-                                                       double* part) {
+__global__ __launch_bounds__(1024) void k_mri_primal(const SliceDev* sl, const int2* items, int nitems, int rpi,
+                                                     double* part) {
   const int li = xcd_item(blockIdx.x, nitems);
   const int2 it = items[li];
   const SliceDev& d = sl[it.x];
-  if (!slice_active(d)) return;
+  if (!slice_active(d) || d.ngr == 1) return;
   using C = Cx<R>;
   const int W = d.W, H = d.H, ngr = d.ngr;
   const size_t n = size_t(W) * H;
   const int ya = it.y, yb = min(H, ya + rpi);
   const C* wp = static_cast<const C*>(d.wpart);
   const R tau = R(d.tau), theta = R(d.theta);
-  C* u = static_cast<C*>(d.u);
-  C* ub = static_cast<C*>(d.ub);
-  const C* px = static_cast<const C*>(d.p);
-  const C* py = px + n;
   double t[2] = {0, 0};
   const int cnt = (yb - ya) * W;
-  for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
+  for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
     const int yy = idx / W;
     const int x = idx - yy * W, y = ya + yy;
     const size_t i = size_t(y) * W + x;
     C w = wp[i];
     for (int g = 1; g < ngr; ++g) w = w + wp[size_t(g) * n + i];
-    if (MODE == M_OP) {
-      static_cast<C*>(d.op_out)[i] = w;
-      continue;
-    }
-    const C uo = u[i];
-    const C div = dmx<C>(px, x, i, W) + dmy<C>(py, y, i, W, H);
-    const C un = uo + tau * (div - w);
-    u[i] = un;
-    ub[i] = (R(1) + theta) * un - theta * uo;
-    if (GAP) t[0] += dabs2(w - div);
-    if (MODE == M_TGV) {
-      const double rv = vstep_pixel<C, C>(d, x, y, tau, theta);
-      if (GAP) t[1] += rv;
-    }
+    if (MODE == M_OP) static_cast<C*>(d.op_out)[i] = w;
+    else mri_primal_pixel<R, MODE, GAP>(d, x, y, w, tau, theta, t);
   }
   if (MODE == M_OP) return;
-  if (GAP) block_sum<2>(t, part + size_t(li) * kGapTerms);
+  if (GAP) block_sum_dyn<2>(t, part + size_t(li) * kGapTerms);
   if (ya == 0 && threadIdx.x == 0) {
     d.ctl[CTL_ITER] += 1;
     d.ctl[CTL_RUPD] += 1;
@@ -1238,11 +1369,13 @@ static void mri_stage_m(const Launch& L, bool gap, int stage) {
   } else if (stage == 2) {
     with_fft_len<R>(L.fft_w, [&](auto nf) {
       constexpr int NF = decltype(nf)::value;
-      PDT_LAUNCH_NT((k_rows_adj<R, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb);
+      if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.gap_rows);
+      else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.gap_rows);
     });
   } else {
-    if (gap) PDT_LAUNCH((k_mri_primal<R, MODE, true>), L.nrows, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
-    else PDT_LAUNCH((k_mri_primal<R, MODE, false>), L.nrows, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
+    if (!L.split_primal) return;
+    if (gap) PDT_LAUNCH_NT((k_mri_primal<R, MODE, true>), L.nrows, L.primal_threads, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
+    else PDT_LAUNCH_NT((k_mri_primal<R, MODE, false>), L.nrows, L.primal_threads, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
   }
 }
 
@@ -1305,9 +1438,10 @@ static void op_adjoint(const Launch& L) {
   });
   with_fft_len<R>(L.fft_w, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;
-    PDT_LAUNCH_NT((k_rows_adj<R, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb);
+    PDT_LAUNCH_NT((k_rows_adj<R, M_OP, false, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.gap_rows);
   });
-  PDT_LAUNCH((k_mri_primal<R, M_OP, false>), L.nrows, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
+  if (L.split_primal)
+    PDT_LAUNCH_NT((k_mri_primal<R, M_OP, false>), L.nrows, L.primal_threads, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
 }
 
 // Raises the dynamic LDS limit of every FFT kernel instantiation to b bytes.
@@ -1325,7 +1459,11 @@ static bool set_lds_attrs(int b) {
     set((const void*)&k_rows_fwd<R, M_TV, true, NR, NF>);
     set((const void*)&k_rows_fwd<R, M_TGV, false, NR, NF>);
     set((const void*)&k_rows_fwd<R, M_TGV, true, NR, NF>);
-    set((const void*)&k_rows_adj<R, NR, NF>);
+    set((const void*)&k_rows_adj<R, M_OP, false, NR, NF>);
+    set((const void*)&k_rows_adj<R, M_TV, false, NR, NF>);
+    set((const void*)&k_rows_adj<R, M_TV, true, NR, NF>);
+    set((const void*)&k_rows_adj<R, M_TGV, false, NR, NF>);
+    set((const void*)&k_rows_adj<R, M_TGV, true, NR, NF>);
     set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF>);
     set((const void*)&k_cols<R, CM_SOLVE, true, NC, NF>);
     set((const void*)&k_cols<R, CM_FWD, false, NC, NF>);

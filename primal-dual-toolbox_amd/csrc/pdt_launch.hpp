@@ -28,7 +28,13 @@ struct Launch {
   size_t lds_cols = 0;           // dynamic LDS bytes of the column kernel
   int fft_w = 0;                 // row FFT length shared by every slice of the launch (0: mixed)
   int fft_h = 0;                 // column FFT length shared by every slice (0: mixed)
+  double* gap_rows = nullptr;    // row-item partial sums indexed by global row item (primal in k_rows_adj)
+  bool split_primal = true;      // some slice has several coil groups: k_mri_primal completes the step
+  int primal_threads = 256;      // block size of k_mri_primal: rpi * W rounded up to 64, at most 1024
 };
+
+// Block size of k_mri_primal covering rows of width W in one pass.
+inline int primal_block(int W) { return W >= 1024 ? 1024 : ((W + 63) / 64) * 64; }
 
 // One full iteration of the solver kind on every active slice.
 void run_iteration(const Launch& L, int kind, int dtype, bool fp16, bool gap);
