@@ -199,6 +199,41 @@ PDT_API int pdt_autotune(pdt_solver** solvers, int count, int reps, int* cb, int
  * solver keeps its own parameters, sizes, coils and stopping. */
 PDT_API int pdt_batch_solve(pdt_solver** solvers, int count, int verbose);
 
+/* ---- NEW: learned warm start ----------------------------------------------
+ * A conv3x3 network (zero padding, stride 1, ReLU between layers) that maps
+ * the zero-filled reconstruction A^*f -- 2 channels (re, im) divided by its
+ * RMS -- to 6 channels: a residual for u (re, im) and v (re/im of v_x, v_y).
+ * The convolutions run as bf16 MFMA implicit GEMMs with fp32 accumulation;
+ * parameters, gradients and Adam state are fp32.  Supported shapes: cin 2,
+ * width 32, cout 6, depth >= 2 layers.  Parameter order: per layer
+ * W[tap][co][ci] (tap = 3 ky + kx), then b[co].  Tensors are flat PDT_F32
+ * arrays in NHWC order (channel fastest) of n images of h x w. */
+typedef struct pdt_cnn pdt_cnn;
+typedef struct pdt_adam {
+  double lr, beta1, beta2, eps;
+} pdt_adam;
+PDT_API int pdt_cnn_create(int cin, int width, int depth, int cout, pdt_cnn** out);
+PDT_API int pdt_cnn_destroy(pdt_cnn* net);
+PDT_API int pdt_cnn_num_params(const pdt_cnn* net, size_t* n);
+/* Uploads the parameters and resets the Adam moments and step count. */
+PDT_API int pdt_cnn_set_params(pdt_cnn* net, const float* host, size_t n);
+PDT_API int pdt_cnn_get_params(const pdt_cnn* net, float* host, size_t n);
+/* y = net(x); x holds n h w cin floats, y n h w cout. */
+PDT_API int pdt_cnn_forward(pdt_cnn* net, const pdt_array* x, pdt_array* y, int n, int h, int w);
+/* Loss 1/(2m) sum (net(x) - t)^2 over the m = n h w cout outputs and its
+ * gradient, kept for pdt_cnn_adam_step and copied to grad_host when given. */
+PDT_API int pdt_cnn_loss_grad(pdt_cnn* net, const pdt_array* x, const pdt_array* t, int n, int h, int w,
+                              double* loss, float* grad_host, size_t n_grad);
+/* One Adam update (Kingma & Ba, Algorithm 1) with the last gradient. */
+PDT_API int pdt_cnn_adam_step(pdt_cnn* net, const pdt_adam* opt);
+/* pdt_cnn_loss_grad followed by pdt_cnn_adam_step. */
+PDT_API int pdt_cnn_train_step(pdt_cnn* net, const pdt_array* x, const pdt_array* t, int n, int h, int w,
+                               const pdt_adam* opt, double* loss);
+/* Warm start of a complex64 TGV-MRI / TGV-with-operator solver: u0 = A^*f,
+ * s = RMS(u0), (r_u, v') = net(u0 / s); sets u = u0 + s r_u and v = s v'
+ * (p, q zeroed) as pdt_solver_set_input0 would. */
+PDT_API int pdt_solver_warm_start(pdt_solver* s, pdt_cnn* net);
+
 #ifdef __cplusplus
 }
 #endif

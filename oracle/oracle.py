@@ -1,7 +1,8 @@
 """ctypes front end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
 
 The oracle is an FP64 C++ restatement of the reference's TV/TGV primal-dual
-solvers and Cartesian MRI operators (``pdt_oracle.hpp``).  Only ``tests/``,
+solvers and Cartesian MRI operators (``pdt_oracle.hpp``), plus the warm-start
+network and Adam (``pdt_cnn_oracle.cpp``).  Only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg import this
 module, always as the checker or the reported CPU baseline, never as part of
 the product path.
@@ -66,6 +67,14 @@ def lib():
         L.pdto_stencil.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p]
         L.pdto_set_threads.argtypes = [ct.c_int]
         L.pdto_max_threads.restype = ct.c_int
+        ip = ct.POINTER(ct.c_int)
+        L.pdto_cnn_num_params.argtypes = [ct.c_int, ip]
+        L.pdto_cnn_forward.argtypes = [ct.c_int, ip, ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                                       ct.c_void_p, ct.c_void_p]
+        L.pdto_cnn_loss_grad.argtypes = [ct.c_int, ip, ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                                         ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_double), ct.c_void_p]
+        L.pdto_adam.argtypes = [ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_double,
+                                ct.c_double, ct.c_double, ct.c_double, ct.c_int]
         _lib = L
     return _lib
 
@@ -179,3 +188,47 @@ def stencil(which: int, a):
     out = np.empty((nout, H, W), a.dtype)
     lib().pdto_stencil(which, int(cplx), W, H, _ptr(a), _ptr(out))
     return out[0] if nout == 1 else out
+
+
+# --------------------------------------------------------------------------
+# warm-start network (pdt_cnn_oracle.cpp)
+# --------------------------------------------------------------------------
+def _chans(ch):
+    ch = [int(c) for c in ch]
+    return len(ch) - 1, (ct.c_int * len(ch))(*ch)
+
+
+def cnn_num_params(ch) -> int:
+    nl, c = _chans(ch)
+    return int(lib().pdto_cnn_num_params(nl, c))
+
+
+def cnn_forward(ch, params, x, bf16: bool = False):
+    """x: (n, h, w, ch[0]) -> (n, h, w, ch[-1]), FP64 (bf16 rounding emulated if asked)."""
+    nl, c = _chans(ch)
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    n, h, w, _ = x.shape
+    y = np.empty((n, h, w, ch[-1]), np.float64)
+    lib().pdto_cnn_forward(nl, c, _ptr(p), n, h, w, int(bf16), _ptr(x), _ptr(y))
+    return y
+
+
+def cnn_loss_grad(ch, params, x, t, bf16: bool = False):
+    """Loss 1/(2m) sum (net(x) - t)^2 and its gradient (parameter order of the network)."""
+    nl, c = _chans(ch)
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    t = np.ascontiguousarray(t, np.float64)
+    n, h, w, _ = x.shape
+    g = np.empty_like(p)
+    loss = ct.c_double()
+    lib().pdto_cnn_loss_grad(nl, c, _ptr(p), n, h, w, int(bf16), _ptr(x), _ptr(t), ct.byref(loss), _ptr(g))
+    return loss.value, g
+
+
+def adam(w, g, m, v, lr, b1, b2, eps, step):
+    """One Adam step in place on FP64 arrays (step counted from 1)."""
+    for a in (w, g, m, v):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    lib().pdto_adam(w.size, _ptr(w), _ptr(g), _ptr(m), _ptr(v), lr, b1, b2, eps, int(step))

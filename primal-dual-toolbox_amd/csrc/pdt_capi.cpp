@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "pdt_capi.h"
+#include "pdt_cnn.hpp"
 #include "pdt_desc.hpp"
 #include "pdt_launch.hpp"
 #include "pdt_tune.hpp"
@@ -109,6 +110,10 @@ struct DevBuf {
   ~DevBuf() { release(); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
@@ -1694,6 +1699,270 @@ int pdt_batch_solve(pdt_solver** solvers, int count, int verbose) {
     for (auto* s : v)
       if (!s) fail(PDT_ERR_ARG, "null solver in batch");
     solve_many(v, verbose);
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// learned warm start
+// ---------------------------------------------------------------------------
+struct pdt_cnn {
+  int nl = 0;
+  std::vector<int> ch;         // nl + 1 channel counts
+  std::vector<size_t> off;     // offset of layer l's W in the parameter vector; b follows it
+  size_t np = 0;
+  DevBuf w, g, m, v;           // fp32 parameters, gradient, Adam moments
+  std::vector<DevBuf> wk, wt;  // packed bf16 weights (forward, backward-data)
+  int64_t step = 0;
+  int64_t cap = 0;             // pixels the activation buffers hold
+  DevBuf xb, y, dza, dzb, part, lpart;
+  std::vector<DevBuf> act;     // hidden activations (bf16), nl - 1 of them
+  hipStream_t stream = nullptr;
+  ~pdt_cnn() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  size_t wsize(int l) const { return size_t(9) * ch[l] * ch[l + 1]; }
+  const float* W(int l) const { return w.as<float>() + off[l]; }
+  const float* b(int l) const { return w.as<float>() + off[l] + wsize(l); }
+
+  void ensure(int64_t P) {
+    if (P <= cap) return;
+    xb.alloc(size_t(P) * ch[0] * 2);
+    for (int l = 0; l + 1 < nl; ++l) act[l].alloc(size_t(P) * ch[l + 1] * 2);
+    y.alloc(size_t(P) * ch[nl] * 4);
+    int cmax = 1;
+    for (int c : ch) cmax = std::max(cmax, c);
+    dza.alloc(size_t(P) * cmax * 2);
+    dzb.alloc(size_t(P) * cmax * 2);
+    size_t pf = 0;
+    for (int l = 0; l < nl; ++l) pf = std::max(pf, cnn_wgrad_part_floats(ch[l], ch[l + 1], 1024));
+    part.alloc(pf * 4);
+    lpart.alloc(1024 * sizeof(double));
+    cap = P;
+  }
+
+  void pack() {
+    for (int l = 0; l < nl; ++l) cnn_pack_weights(stream, W(l), ch[l], ch[l + 1], wk[l].p, l > 0 ? wt[l].p : nullptr);
+  }
+
+  // y = net(x) for P = n h w pixels (x: fp32 NHWC on the device)
+  void forward(const float* x, int n, int h, int wd) {
+    const int64_t P = int64_t(n) * h * wd;
+    ensure(P);
+    cnn_to_bf16(stream, x, xb.p, P * ch[0]);
+    const void* in = xb.p;
+    for (int l = 0; l < nl; ++l) {
+      const bool last = l + 1 == nl;
+      void* out = last ? y.p : act[l].p;
+      cnn_conv_forward(stream, ch[l], ch[l + 1], !last, in, wk[l].p, b(l), out, n, h, wd);
+      in = out;
+    }
+    HIPC(hipGetLastError());
+  }
+
+  // loss and gradient (into g) of the last forward against t
+  double backward(const float* t, int n, int h, int wd) {
+    const int64_t P = int64_t(n) * h * wd, M = P * ch[nl];
+    const int nb = cnn_mse_blocks(M);
+    void* dz = dza.p;
+    void* dzn = dzb.p;
+    cnn_mse(stream, y.as<float>(), t, M, dz, lpart.as<double>(), nb);
+    const int nblk = cnn_wgrad_blocks(n, h, wd);
+    for (int l = nl - 1; l >= 0; --l) {
+      const void* in = l == 0 ? xb.p : act[l - 1].p;
+      cnn_conv_wgrad(stream, ch[l], ch[l + 1], in, dz, part.as<float>(), nblk, g.as<float>() + off[l], n, h, wd);
+      if (l > 0) {
+        cnn_conv_backward_data(stream, ch[l], ch[l + 1], dz, wt[l].p, act[l - 1].p, dzn, n, h, wd);
+        std::swap(dz, dzn);
+      }
+    }
+    HIPC(hipGetLastError());
+    std::vector<double> lp(nb);
+    HIPC(hipMemcpyAsync(lp.data(), lpart.p, nb * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPC(hipStreamSynchronize(stream));
+    double ss = 0;
+    for (double x : lp) ss += x;
+    return ss / (2.0 * double(M));
+  }
+
+  void adam(const pdt_adam& o) {
+    ++step;
+    const double c1 = 1.0 - std::pow(o.beta1, double(step)), c2 = 1.0 - std::pow(o.beta2, double(step));
+    cnn_adam(stream, w.as<float>(), g.as<float>(), m.as<float>(), v.as<float>(), int64_t(np), float(o.lr),
+             float(o.beta1), float(o.beta2), float(o.eps), float(c1), float(c2));
+    pack();
+    HIPC(hipGetLastError());
+  }
+};
+
+namespace {
+const float* cnn_tensor(const pdt_array* a, int64_t want, const char* what) {
+  if (!a) fail(PDT_ERR_ARG, std::string("null ") + what);
+  if (a->dtype != PDT_F32) fail(PDT_ERR_ARG, std::string(what) + " must be PDT_F32");
+  if (int64_t(a->numel()) != want)
+    fail(PDT_ERR_SIZE, std::string(what) + " holds " + std::to_string(a->numel()) + " floats, expected " +
+                           std::to_string(want));
+  return static_cast<const float*>(a->ptr);
+}
+void cnn_dims(int n, int h, int w) {
+  if (n <= 0 || h <= 0 || w <= 0) fail(PDT_ERR_ARG, "n, h, w must be positive");
+}
+}  // namespace
+
+extern "C" {
+
+int pdt_cnn_create(int cin, int width, int depth, int cout, pdt_cnn** out) {
+  return guard([&] {
+    if (!out) fail(PDT_ERR_ARG, "null output");
+    if (depth < 2) fail(PDT_ERR_ARG, "depth must be at least 2");
+    auto net = std::make_unique<pdt_cnn>();
+    net->nl = depth;
+    net->ch.assign(depth + 1, width);
+    net->ch[0] = cin;
+    net->ch[depth] = cout;
+    for (int l = 0; l < depth; ++l)
+      if (!cnn_layer_supported(net->ch[l], net->ch[l + 1]))
+        fail(PDT_ERR_UNSUPPORTED, "no kernels for a " + std::to_string(net->ch[l]) + " -> " +
+                                      std::to_string(net->ch[l + 1]) + " conv layer (supported: 2-32-...-32-6)");
+    for (int l = 0; l < depth; ++l) {
+      net->off.push_back(net->np);
+      net->np += net->wsize(l) + size_t(net->ch[l + 1]);
+    }
+    HIPC(hipStreamCreate(&net->stream));
+    net->w.alloc(net->np * 4);
+    net->g.alloc(net->np * 4);
+    net->m.alloc(net->np * 4);
+    net->v.alloc(net->np * 4);
+    net->wk.resize(depth);
+    net->wt.resize(depth);
+    net->act.resize(depth - 1);
+    for (int l = 0; l < depth; ++l) {
+      net->wk[l].alloc(cnn_wk_elems(net->ch[l], net->ch[l + 1]) * 2);
+      if (l > 0) net->wt[l].alloc(cnn_wt_elems(net->ch[l], net->ch[l + 1]) * 2);
+    }
+    run_zero(net->stream, net->w.p, net->np * 4);
+    run_zero(net->stream, net->m.p, net->np * 4);
+    run_zero(net->stream, net->v.p, net->np * 4);
+    net->pack();
+    HIPC(hipStreamSynchronize(net->stream));
+    *out = net.release();
+  });
+}
+
+int pdt_cnn_destroy(pdt_cnn* net) {
+  return guard([&] { delete net; });
+}
+
+int pdt_cnn_num_params(const pdt_cnn* net, size_t* n) {
+  return guard([&] {
+    if (!net || !n) fail(PDT_ERR_ARG, "null argument");
+    *n = net->np;
+  });
+}
+
+int pdt_cnn_set_params(pdt_cnn* net, const float* host, size_t n) {
+  return guard([&] {
+    if (!net || !host) fail(PDT_ERR_ARG, "null argument");
+    if (n != net->np) fail(PDT_ERR_SIZE, "expected " + std::to_string(net->np) + " parameters, got " + std::to_string(n));
+    HIPC(hipMemcpy(net->w.p, host, n * 4, hipMemcpyHostToDevice));
+    run_zero(net->stream, net->m.p, n * 4);
+    run_zero(net->stream, net->v.p, n * 4);
+    net->step = 0;
+    net->pack();
+    HIPC(hipStreamSynchronize(net->stream));
+  });
+}
+
+int pdt_cnn_get_params(const pdt_cnn* net, float* host, size_t n) {
+  return guard([&] {
+    if (!net || !host) fail(PDT_ERR_ARG, "null argument");
+    if (n != net->np) fail(PDT_ERR_SIZE, "expected " + std::to_string(net->np) + " parameters, got " + std::to_string(n));
+    HIPC(hipStreamSynchronize(net->stream));
+    HIPC(hipMemcpy(host, net->w.p, n * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int pdt_cnn_forward(pdt_cnn* net, const pdt_array* x, pdt_array* y, int n, int h, int w) {
+  return guard([&] {
+    if (!net) fail(PDT_ERR_ARG, "null network");
+    cnn_dims(n, h, w);
+    const int64_t P = int64_t(n) * h * w;
+    const float* xp = cnn_tensor(x, P * net->ch[0], "x");
+    float* yp = const_cast<float*>(cnn_tensor(y, P * net->ch[net->nl], "y"));
+    net->forward(xp, n, h, w);
+    HIPC(hipMemcpyAsync(yp, net->y.p, size_t(P) * net->ch[net->nl] * 4, hipMemcpyDeviceToDevice, net->stream));
+    HIPC(hipStreamSynchronize(net->stream));
+  });
+}
+
+int pdt_cnn_loss_grad(pdt_cnn* net, const pdt_array* x, const pdt_array* t, int n, int h, int w, double* loss,
+                      float* grad_host, size_t n_grad) {
+  return guard([&] {
+    if (!net) fail(PDT_ERR_ARG, "null network");
+    cnn_dims(n, h, w);
+    const int64_t P = int64_t(n) * h * w;
+    const float* xp = cnn_tensor(x, P * net->ch[0], "x");
+    const float* tp = cnn_tensor(t, P * net->ch[net->nl], "t");
+    if (grad_host && n_grad != net->np) fail(PDT_ERR_SIZE, "gradient buffer must hold every parameter");
+    net->forward(xp, n, h, w);
+    const double l = net->backward(tp, n, h, w);
+    if (loss) *loss = l;
+    if (grad_host) HIPC(hipMemcpy(grad_host, net->g.p, net->np * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int pdt_cnn_adam_step(pdt_cnn* net, const pdt_adam* opt) {
+  return guard([&] {
+    if (!net || !opt) fail(PDT_ERR_ARG, "null argument");
+    net->adam(*opt);
+    HIPC(hipStreamSynchronize(net->stream));
+  });
+}
+
+int pdt_cnn_train_step(pdt_cnn* net, const pdt_array* x, const pdt_array* t, int n, int h, int w,
+                       const pdt_adam* opt, double* loss) {
+  return guard([&] {
+    if (!net || !opt) fail(PDT_ERR_ARG, "null argument");
+    cnn_dims(n, h, w);
+    const int64_t P = int64_t(n) * h * w;
+    const float* xp = cnn_tensor(x, P * net->ch[0], "x");
+    const float* tp = cnn_tensor(t, P * net->ch[net->nl], "t");
+    net->forward(xp, n, h, w);
+    const double l = net->backward(tp, n, h, w);
+    if (loss) *loss = l;
+    net->adam(*opt);
+    HIPC(hipStreamSynchronize(net->stream));
+  });
+}
+
+int pdt_solver_warm_start(pdt_solver* s, pdt_cnn* net) {
+  return guard([&] {
+    if (!s || !net) fail(PDT_ERR_ARG, "null argument");
+    if (s->kind != PDT_SOLVER_TGV_MRI && s->kind != PDT_SOLVER_TGV_OP)
+      fail(PDT_ERR_UNSUPPORTED, "the warm start predicts (u, v) of the TGV operator solvers");
+    if (s->dtype != PDT_C64) fail(PDT_ERR_UNSUPPORTED, "the warm start runs on complex64 solvers");
+    if (net->ch[0] != 2 || net->ch[net->nl] != 6) fail(PDT_ERR_ARG, "the warm-start network maps 2 to 6 channels");
+    prepare_solve(s, 0);
+    const int W = s->op->W, H = s->op->H;
+    const int64_t n = int64_t(W) * H;
+    DevBuf u0(size_t(n) * 8), x(size_t(n) * 2 * 4), u(size_t(n) * 8), v(size_t(n) * 16), part(cnn_sumsq_blocks() * 8);
+    s->op->apply(false, s->f->ptr, u0.p, s->stream);
+    cnn_ws_sumsq(s->stream, u0.p, n, part.as<double>());
+    std::vector<double> hp(cnn_sumsq_blocks());
+    HIPC(hipMemcpyAsync(hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
+    double ss = 0;
+    for (double q : hp) ss += q;
+    const double scale = ss > 0 ? std::sqrt(ss / double(n)) : 1.0;
+    cnn_ws_pack(s->stream, u0.p, n, float(1.0 / scale), x.as<float>());
+    HIPC(hipStreamSynchronize(s->stream));
+    net->forward(x.as<float>(), 1, H, W);
+    cnn_ws_unpack(net->stream, x.as<float>(), net->y.as<float>(), n, float(scale), u.p, v.p);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(net->stream));
+    set_input0(s, u.p, v.p, W, H);
   });
 }
 

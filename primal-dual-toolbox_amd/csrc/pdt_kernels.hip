@@ -396,14 +396,15 @@ constexpr int ct_nstages(int n) {
   return s;
 }
 
-template <class R, int RR>
+// TW: stage twiddles present (false for the first stage, where they are 1).
+template <class R, int RR, bool TW>
 __device__ __forceinline__ void butterfly_ct(const Cx<R>* in, Cx<R>* out, int ts, const Cx<R>* tw, bool inv,
                                              int nr, int p) {
   if constexpr (RR == 2 || RR == 3 || RR == 4 || RR == 5 || RR == 8) {
     Cx<R> x[RR];
     x[0] = in[0];
 #pragma unroll
-    for (int j = 1; j < RR; ++j) x[j] = in[j * nr] * twd(tw, j * ts, inv);
+    for (int j = 1; j < RR; ++j) x[j] = TW ? in[j * nr] * twd(tw, j * ts, inv) : in[j * nr];
     Dft<R, RR>::run(x, inv ? R(1) : R(-1));
 #pragma unroll
     for (int q = 0; q < RR; ++q) out[q * p] = x[q];
@@ -424,7 +425,7 @@ __device__ __forceinline__ Cx<R>* fft_ct_stage(Cx<R>* a, Cx<R>* b, int nseq, con
       const int seq = t / nr;
       const int i = t - seq * nr;
       const int k = i % P;
-      butterfly_ct<R, r>(a + seq * N + i, b + seq * N + (i - k) * r + k, k * tws, tw, INV, nr, P);
+      butterfly_ct<R, r, (P > 1)>(a + seq * N + i, b + seq * N + (i - k) * r + k, k * tws, tw, INV, nr, P);
     }
     __syncthreads();
     return fft_ct_stage<R, N, S + 1, P * r, NT, INV>(b, a, nseq, tw);

@@ -30,6 +30,8 @@ EXPORTS = [
     "pdt_solver_set_operator", "pdt_solver_set_noisy_data", "pdt_solver_set_input0",
     "pdt_solver_solve", "pdt_solver_get_result", "pdt_solver_get_v", "pdt_solver_get_stats",
     "pdt_solver_destroy", "pdt_solver_profile", "pdt_autotune", "pdt_batch_solve",
+    "pdt_cnn_create", "pdt_cnn_destroy", "pdt_cnn_num_params", "pdt_cnn_set_params", "pdt_cnn_get_params",
+    "pdt_cnn_forward", "pdt_cnn_loss_grad", "pdt_cnn_adam_step", "pdt_cnn_train_step", "pdt_solver_warm_start",
 ]
 
 
@@ -48,6 +50,10 @@ class PdtStats(ct.Structure):
 
 class PdtKernelProf(ct.Structure):
     _fields_ = [("name", ct.c_char * 32), ("ms", ct.c_double), ("bytes", ct.c_double)]
+
+
+class PdtAdam(ct.Structure):
+    _fields_ = [("lr", ct.c_double), ("beta1", ct.c_double), ("beta2", ct.c_double), ("eps", ct.c_double)]
 
 
 class PdtError(RuntimeError):
@@ -105,6 +111,17 @@ def lib():
         ip = ct.POINTER(ct.c_int)
         L.pdt_autotune.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ip, ip, ip, dp]
         L.pdt_batch_solve.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int]
+        fp, szp = ct.POINTER(ct.c_float), ct.POINTER(ct.c_size_t)
+        L.pdt_cnn_create.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(vp)]
+        L.pdt_cnn_destroy.argtypes = [vp]
+        L.pdt_cnn_num_params.argtypes = [vp, szp]
+        L.pdt_cnn_set_params.argtypes = [vp, fp, ct.c_size_t]
+        L.pdt_cnn_get_params.argtypes = [vp, fp, ct.c_size_t]
+        L.pdt_cnn_forward.argtypes = [vp, vp, vp, ct.c_int, ct.c_int, ct.c_int]
+        L.pdt_cnn_loss_grad.argtypes = [vp, vp, vp, ct.c_int, ct.c_int, ct.c_int, dp, fp, ct.c_size_t]
+        L.pdt_cnn_adam_step.argtypes = [vp, ct.POINTER(PdtAdam)]
+        L.pdt_cnn_train_step.argtypes = [vp, vp, vp, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(PdtAdam), dp]
+        L.pdt_solver_warm_start.argtypes = [vp, vp]
         _lib = L
     return _lib
 
