@@ -1,14 +1,17 @@
 #!/usr/bin/env python3
 """Benchmark: time-to-gap of the Cartesian TGV-MRI solver on one MI355X.
 
-Workload (one "step" = one cold-start solve to relative primal-dual gap 1e-4):
+Workload (one "step" = one solve to relative primal-dual gap 1e-4, cold and
+warm-started):
   320 x 320 complex64 image (ellipse phantom, quadratic ramp, polynomial
   phase), 15 coils (Gaussian bumps on a circle, smooth phase, sum |S|^2 = 1),
   read-out oversampled k-space {320 PE, 640 FE}, random 4x PE-line mask with
   24 fully sampled centre lines, complex noise at 30 dB SNR,
   MriCartesianRemoveROOSOperator, TgvMriOptimizer with alpha1 = 1e-5 ||f||,
   alpha0 = 2 alpha1, reduction 1, gap evaluated every 50 iterations.
-  Cold start u0 = A^* f (v = p = q = r = 0), as the reference's solve().
+  Cold start u0 = A^* f (v = p = q = r = 0), as the reference's solve();
+  warm start (u0, v0) from the trained network (warmstart.py), its inference
+  inside the timed region.
 
 Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
 """
@@ -79,6 +82,64 @@ def cpu_baseline(case, a1, gpu_iters: int, sample_iters: int, threads: int):
                    f"OpenMP oracle ({per_iter * 1e3:.1f} ms/iteration), scaled to the GPU's "
                    f"{gpu_iters} iterations to rel. gap {GAP_TOL:g}"),
     }
+
+
+def warm_start_steps(opt, case, steps: int, warmup: int):
+    """Time-to-gap from the network's (u, v): warm_start (A^*f, RMS, network,
+    set_input0) and solve, timed together.  None without trained weights."""
+    from primaldualtoolbox import warmstart as ws
+    if not os.path.exists(ws.DEFAULT_WEIGHTS):
+        return None
+    net = ws.WarmStartCNN.load()
+    out = {"seconds": [], "iters": [], "rel_gap": [], "network_ms": []}
+    for k in range(warmup + steps):
+        opt.setNoisyData(case["kspace"])  # untimed: host->device upload of f, r = 0
+        t0 = time.perf_counter()
+        net.warm_start(opt)
+        t1 = time.perf_counter()
+        opt.solve()
+        t2 = time.perf_counter()
+        st = opt.getStats()
+        if not st["converged"]:
+            raise RuntimeError(f"warm-started solve did not reach rel. gap {GAP_TOL}: {st}")
+        if k >= warmup:
+            out["seconds"].append(t2 - t0)
+            out["iters"].append(int(st["iters"]))
+            out["rel_gap"].append(float(st["rel_gap"]))
+            out["network_ms"].append((t1 - t0) * 1e3)
+        log(f"[bench] warm {'step' if k >= warmup else 'warmup'}: {t2 - t0:.4f} s, {st['iters']} iterations")
+    out["u"] = opt.getResult().copy()
+    return out
+
+
+def extra_cnn(reps: int = 5) -> dict:
+    """Warm-start network kernels alone: inference on one 320 x 320 slice and
+    one Adam training step at batch 16 (random data), with MFMA TFLOP/s."""
+    from primaldualtoolbox import _capi as C
+    from primaldualtoolbox import warmstart as ws
+    net = ws.WarmStartCNN(seed=0)
+    ch = net.channels
+    macs = sum(9 * a * b for a, b in zip(ch[:-1], ch[1:]))  # per pixel
+    rng = np.random.default_rng(0)
+    res = {"network": ch, "parameters": net.n_params}
+    for name, n in (("inference_1x320x320", 1), ("train_step_16x320x320", 16)):
+        x = C.DeviceArray.from_numpy(rng.standard_normal(n * 320 * 320 * 2).astype(np.float32))
+        t = C.DeviceArray.from_numpy(rng.standard_normal(n * 320 * 320 * 6).astype(np.float32))
+        y = C.DeviceArray.empty(C.PDT_F32, [n * 320 * 320 * 6])
+
+        def run():
+            if n == 1:
+                C.check(C.lib().pdt_cnn_forward(net._h, x.h, y.h, 1, 320, 320))
+            else:
+                net.train_step(x, t, lr=1e-3, shape=(n, 320, 320))
+        run()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        dt = (time.perf_counter() - t0) / reps
+        flops = 2.0 * macs * n * 320 * 320 * (1 if n == 1 else 3)  # training: forward + about 2x backward
+        res[name] = {"ms": dt * 1e3, "TFLOPs": flops / dt / 1e12}
+    return res
 
 
 def _oracle():
@@ -282,6 +343,7 @@ def main():
                     help="skip the TV 256^2, TGV 512^2 and batched-slices side measurements")
     ap.add_argument("--batch", type=int, default=64, help="slices in the batched side measurement")
     ap.add_argument("--gen-workers", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-warm", action="store_true", help="report the cold start only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -350,7 +412,10 @@ def main():
         one_step()
         step_s.append(time.perf_counter() - t0)
         log(f"[bench] step {k}: {step_s[-1]:.4f} s, {iters[-1]} iterations, rel gap {gaps[-1]:.3e}")
-    value = float(np.mean(step_s))
+    cold_s = float(np.mean(step_s))
+    u_cold = opt.getResult().copy()
+    warm = None if args.no_warm else warm_start_steps(opt, case, args.steps, args.warmup)
+    value = float(np.mean(warm["seconds"])) if warm else cold_s
 
     # per-kernel durations (hipEvents on the solver stream) and algorithmic bytes
     prof = opt.profileKernels(args.profile_reps)
@@ -374,7 +439,8 @@ def main():
     extra = None
     if not args.no_extra:
         extra = {}
-        for key, fn in (("tv_256", lambda: extra_tv256(3, args.cpu_threads, args.no_cpu)),
+        for key, fn in (("cnn", extra_cnn),
+                        ("tv_256", lambda: extra_tv256(3, args.cpu_threads, args.no_cpu)),
                         ("tgv_512", lambda: extra_tgv512(3, args.cpu_threads, args.no_cpu)),
                         (f"batch_{args.batch}", lambda: extra_batch64(args.batch, args.gen_workers,
                                                                       args.cpu_threads, args.no_cpu))):
@@ -386,7 +452,7 @@ def main():
             log(f"[bench] {key}: {extra[key]}")
 
     line = {
-        "metric": "time-to-gap 1e-4 (s), 15-coil 320^2 Cartesian TGV, cold start",
+        "metric": "time-to-gap 1e-4 (s), 15-coil 320^2 Cartesian TGV, warm vs cold",
         "value": value,
         "unit": "s",
         "n_gpus": 1,
@@ -401,7 +467,16 @@ def main():
         "config": {
             "workload": "Cartesian TGV-MRI 320x320 complex64, 15 coils, ROOS grid 320 PE x 640 FE, random 4x "
                         "PE-line mask (24 centre lines), 30 dB SNR, alpha1=1e-5||f||, alpha0=2 alpha1, "
-                        "rel. partial gap 1e-4 checked every 50 iterations, cold start u0=A^*f",
+                        "rel. partial gap 1e-4 checked every 50 iterations; value = warm start (network "
+                        "(u, v) from A^*f, inference timed) when trained weights exist, else cold start u0=A^*f",
+            "start": "warm" if warm else "cold",
+            "cold_start_s": cold_s,
+            "warm_start_s": float(np.mean(warm["seconds"])) if warm else None,
+            "warm_speedup": cold_s / float(np.mean(warm["seconds"])) if warm else None,
+            "iterations_cold": int(np.median(iters)),
+            "iterations_warm": int(np.median(warm["iters"])) if warm else None,
+            "warm_network_ms": float(np.median(warm["network_ms"])) if warm else None,
+            "warm_vs_cold_rel_l2_u": _rel(warm["u"], u_cold) if warm else None,
             "iterations_to_gap": int(np.median(iters)),
             "final_rel_gap": float(np.median(gaps)),
             "device_loop_ms": float(np.median(loop_ms)),

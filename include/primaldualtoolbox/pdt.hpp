@@ -265,6 +265,11 @@ class SolverBase {
     check(pdt_solver_set_noisy_data(h_, f->handle()));
   }
   void setInput0(const InputType& u0) { check(pdt_solver_set_input0(h_, u0.handle(), nullptr)); }
+  // NEW: also warm-starts v (TGV solvers), v0 of size {W, H, 2}.
+  void setInput0(const InputType& u0, const DeviceArray<pixel_type, 3>& v0) {
+    check(pdt_solver_set_input0(h_, u0.handle(), v0.handle()));
+  }
+  pdt_solver* handle() const { return h_; }
   void solve(bool verbose = false) {
     pdt_params c = to_c(Kind, params_);
     check(pdt_solver_set_params(h_, &c));
@@ -323,5 +328,38 @@ template <class InputType, class OutputType>
 class TgvOptimizerWithOp
     : public detail::OpSolverBase<InputType, OutputType, PDT_SOLVER_TGV_OP,
                                   TgvParameters<typename real_of<typename InputType::pixel_type>::type>> {};
+
+// NEW: learned warm start of the complex64 TGV operator solvers (no
+// counterpart in the reference): a conv3x3 network 2 -> width -> ... -> 6
+// predicting (u, v) from A^*f; see pdt_cnn_* in pdt_capi.h.
+class WarmStartNetwork {
+ public:
+  explicit WarmStartNetwork(int depth = 5, int width = 32) { check(pdt_cnn_create(2, width, depth, 6, &h_)); }
+  ~WarmStartNetwork() {
+    if (h_) pdt_cnn_destroy(h_);
+  }
+  WarmStartNetwork(const WarmStartNetwork&) = delete;
+  void operator=(const WarmStartNetwork&) = delete;
+  size_t numParams() const {
+    size_t n = 0;
+    check(pdt_cnn_num_params(h_, &n));
+    return n;
+  }
+  void setParams(const std::vector<float>& p) { check(pdt_cnn_set_params(h_, p.data(), p.size())); }
+  std::vector<float> getParams() const {
+    std::vector<float> p(numParams());
+    check(pdt_cnn_get_params(h_, p.data(), p.size()));
+    return p;
+  }
+  // Sets (u0, v0) of a configured TgvMriOptimizer / TgvOptimizerWithOp.
+  template <class Solver>
+  void warmStart(Solver& s) {
+    check(pdt_solver_warm_start(s.handle(), h_));
+  }
+  pdt_cnn* handle() const { return h_; }
+
+ private:
+  pdt_cnn* h_ = nullptr;
+};
 
 }  // namespace pdt

@@ -193,11 +193,24 @@ def batch_case(seed: int):
     return case
 
 
+def process_pool(workers: int):
+    """A process pool whose workers start fresh interpreters (spawn), so that
+    no worker inherits a parent that has already opened the GPU."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    return cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn"))
+
+
 def batch_cases(seeds, workers: int = 16):
     """Generates batch_case(seed) for each seed in a process pool."""
-    import concurrent.futures as cf
     seeds = list(seeds)
     if workers <= 1 or len(seeds) <= 1:
         return [batch_case(s) for s in seeds]
-    with cf.ProcessPoolExecutor(max_workers=min(workers, len(seeds))) as ex:
+    with process_pool(min(workers, len(seeds))) as ex:
         return list(ex.map(batch_case, seeds))
+
+
+def cfg3_case(seed: int):
+    """A slice like the 320 x 320, 15-coil benchmark workload with its own seed
+    (4x random lines, 24 centre lines, 30 dB)."""
+    return mri_case(seed=seed, n=320, coils=15, accel=4.0, center=24, snr_db=30.0, roos=True)

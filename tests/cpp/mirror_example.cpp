@@ -74,7 +74,32 @@ int main() {
     op.addConstant(M, pdt::Size<2>{10, 10});
     const double adj = op.adjointnessCheck();
     std::printf("%s adjointness rel. error = %.3e\n", op.print().c_str(), adj);
-    return (err < 1e-5 && adj < 1e-5) ? 0 : 1;
+
+    // warm start with an all-zero network: u0 = A^*f exactly, v0 = 0
+    auto mop = std::make_shared<pdt::MriCartesianOperator<Arr2c, Arr3c>>();
+    mop->addConstant(S, pdt::Size<3>{10, 10, 5});
+    mop->addConstant(M, pdt::Size<2>{10, 10});
+    std::vector<std::complex<float>> fk(10 * 10 * 5);
+    for (size_t i = 0; i < fk.size(); ++i) fk[i] = {M[i % 100] * nd(rng), M[i % 100] * nd(rng)};
+    auto fdev = std::make_shared<Arr3c>(pdt::Size<3>{10, 10, 5});
+    fdev->upload(fk);
+    Arr2c atf(pdt::Size<2>{10, 10});
+    mop->adjoint(*fdev, atf);
+    pdt::TgvMriOptimizer<Arr2c, Arr3c> mri;
+    mri.setOperator(mop);
+    mri.setNoisyData(fdev);
+    pdt::WarmStartNetwork net;
+    net.setParams(std::vector<float>(net.numParams(), 0.f));
+    net.warmStart(mri);
+    const std::vector<std::complex<float>> u0 = mri.getResult()->download(), a = atf.download();
+    double wnum = 0, wden = 0;
+    for (size_t i = 0; i < a.size(); ++i) {
+      wnum += std::norm(u0[i] - a[i]);
+      wden += std::norm(a[i]);
+    }
+    const double werr = std::sqrt(wnum / wden);
+    std::printf("warm start (zero network, %zu parameters): rel. difference to A^*f = %.3e\n", net.numParams(), werr);
+    return (err < 1e-5 && adj < 1e-5 && werr < 1e-6) ? 0 : 1;
   } catch (const pdt::Exception& e) {
     std::printf("pdt::Exception (%d): %s\n", e.code(), e.what());
     return 2;

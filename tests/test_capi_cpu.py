@@ -140,3 +140,12 @@ def test_cpp_mirror_compiles():
 def _lib_path():
     from primaldualtoolbox import _capi as C
     return C.LIB_PATH
+
+
+def test_committed_warm_start_weights_match_the_network():
+    from primaldualtoolbox import warmstart as ws
+    with np.load(ws.DEFAULT_WEIGHTS, allow_pickle=False) as z:
+        depth, width, p = int(z["depth"]), int(z["width"]), z["params"]
+    assert (depth, width) == (5, 32)
+    assert p.dtype == np.float32 and p.size == ws.num_params(ws.layer_channels(depth, width))
+    assert np.isfinite(p).all()
