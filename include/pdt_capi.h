@@ -169,9 +169,12 @@ PDT_API int pdt_solver_get_v(pdt_solver* s, pdt_array** v);
 PDT_API int pdt_solver_get_stats(const pdt_solver* s, pdt_stats* st);
 PDT_API int pdt_solver_destroy(pdt_solver* s);
 
-/* NEW: per-kernel timing of a configured solver.  Runs `reps` iterations on
- * the solver's stream with a hipEvent pair around every fused kernel launch;
- * reported per kernel: the average launch duration and the algorithmic HBM
+/* NEW: per-kernel timing of a configured solver.  A hipGraph of `reps`
+ * iterations (stages in solve order) is timed with a hipEvent pair on the
+ * solver's stream; that time per iteration is divided among the stages in
+ * proportion to event pairs around their direct launches in iteration order
+ * (an event pair adds the same dispatch delay to every launch).  Reported
+ * per kernel: the duration of one launch in the loop and the algorithmic HBM
  * bytes of one launch (every distinct element the launch reads or writes,
  * once, at storage precision).  Stages that launch nothing for this solver
  * (the separate primal kernel when every slice has one coil group) are

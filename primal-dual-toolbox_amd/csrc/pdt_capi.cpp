@@ -1587,10 +1587,32 @@ int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, int cap, i
     e.sv = {s};
     e.build();
     e.reset_ctl();
-    // every stage timed inside real iterations (stage order as in a solve),
-    // an event pair around each launch
+    // The iteration as a solve runs it: a hipGraph of reps iterations, timed
+    // with an event pair on the solver stream (t_it per iteration).  Each
+    // stage's share of it comes from event pairs around its direct launches
+    // in iteration order (e_st); an event pair adds a fixed dispatch delay to
+    // every launch, so the stage duration reported is t_it * e_st / sum(e).
     const int ns = num_stages(e.kind);
-    e.iteration(false);  // warm the code objects
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    hipGraph_t graph;
+    HIPC(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < reps; ++r) e.iteration(false);
+    HIPC(hipStreamEndCapture(e.stream, &graph));
+    hipGraphExec_t ge;
+    HIPC(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    HIPC(hipGraphLaunch(ge, e.stream));  // warm-up replay
+    HIPC(hipEventRecord(e0, e.stream));
+    HIPC(hipGraphLaunch(ge, e.stream));
+    HIPC(hipEventRecord(e1, e.stream));
+    HIPC(hipEventSynchronize(e1));
+    (void)hipGraphExecDestroy(ge);
+    float loop_ms = 0;
+    HIPC(hipEventElapsedTime(&loop_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     std::vector<hipEvent_t> ev(size_t(2) * ns * reps);
     for (auto& x : ev) HIPC(hipEventCreate(&x));
     for (int r = 0; r < reps; ++r)
@@ -1601,23 +1623,29 @@ int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, int cap, i
       }
     HIPC(hipStreamSynchronize(e.stream));
     HIPC(hipGetLastError());
-    int k = 0;
-    for (int st = 0; st < ns && k < cap; ++st) {
-      const double bytes = e.stage_bytes(st);
-      if (bytes <= 0) continue;  // a stage without launches (primal step done inside k_rows_adj)
-      double tot = 0;
+    std::vector<double> share(ns, 0.0);
+    double sum = 0;
+    for (int st = 0; st < ns; ++st) {
+      if (e.stage_bytes(st) <= 0) continue;  // a stage without launches (primal step inside k_rows_adj)
       for (int r = 0; r < reps; ++r) {
         float ms = 0;
         HIPC(hipEventElapsedTime(&ms, ev[(size_t(r) * ns + st) * 2], ev[(size_t(r) * ns + st) * 2 + 1]));
-        tot += ms;
+        share[st] += ms;
       }
+      sum += share[st];
+    }
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    const double t_it = double(loop_ms) / reps;
+    int k = 0;
+    for (int st = 0; st < ns && k < cap; ++st) {
+      const double bytes = e.stage_bytes(st);
+      if (bytes <= 0) continue;
       std::snprintf(out[k].name, sizeof(out[k].name), "%s", stage_name(e.kind, st));
-      out[k].ms = tot / reps;
+      out[k].ms = sum > 0 ? t_it * share[st] / sum : 0.0;
       out[k].bytes = bytes;
       ++k;
     }
     *n = k;
-    for (auto& x : ev) (void)hipEventDestroy(x);
   });
 }
 

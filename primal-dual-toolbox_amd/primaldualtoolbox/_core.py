@@ -232,9 +232,12 @@ class _Optimizer:
         return {k: getattr(st, k) for k, _ in C.PdtStats._fields_}
 
     def profileKernels(self, reps: int = 20) -> list:
-        """Per fused kernel: name, average ms per launch (hipEvents on the
-        solver stream) and algorithmic bytes per launch.  Advances the state
-        arbitrarily, so call it after the solves whose results are used."""
+        """Per fused kernel: name, ms per launch inside the graph-replayed
+        iteration loop (the loop time per iteration, from hipEvents around a
+        graph of reps iterations, divided in proportion to event pairs around
+        each kernel's direct launches) and algorithmic bytes per launch.
+        Advances the state arbitrarily, so call it after the solves whose
+        results are used."""
         self._push_params()
         out = (C.PdtKernelProf * 8)()
         n = ct.c_int()
