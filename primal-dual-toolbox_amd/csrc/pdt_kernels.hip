@@ -474,6 +474,11 @@ __device__ __forceinline__ P dmy(const S* a, int y, size_t i, int W, int H) {
   return (y > 0) ? ((y < H - 1) ? L::get(a, i) - L::get(a, i - W) : -L::get(a, i - W)) : L::get(a, i);
 }
 
+// 1 / x: the hardware reciprocal (within 1 ulp) in single precision, a
+// correctly rounded division in double precision.
+template <class R> __device__ __forceinline__ R rcp(R x) { return R(1) / x; }
+template <> __device__ __forceinline__ float rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <class R>
 __device__ __forceinline__ R sched_alpha(const SliceDev& d, double a_start, double a_end, int k) {
   if (!d.schedule) return R(a_start);
@@ -485,9 +490,10 @@ __device__ __forceinline__ R sched_alpha(const SliceDev& d, double a_start, doub
 // Dual ascent + projection of p (and q) at one pixel, with optional gap terms
 // t[0] += |grad ub - vb|, t[1] += |E vb|_F, t[2] += Re<grad ub - vb, p> +
 // Re<E vb, q>_F (TGV_dual_p_kernel / TGV_dual_q_kernel / TV_dual_p_kernel).
+// ia0, ia1 = 1 / alpha0, 1 / alpha1 (TGV only).
 template <class P, class S, bool TGV, bool GAP>
 __device__ __forceinline__ void dual_pixel(const SliceDev& d, int x, int y, typename RealOf<P>::T sigma,
-                                           typename RealOf<P>::T a0, typename RealOf<P>::T a1,
+                                           typename RealOf<P>::T ia0, typename RealOf<P>::T ia1,
                                            double* t) {
   using R = typename RealOf<P>::T;
   using L = Ld<P, S>;
@@ -516,7 +522,7 @@ __device__ __forceinline__ void dual_pixel(const SliceDev& d, int x, int y, type
     const P byy = L::get(qyy, i) + sigma * eyy;
     const P bxy = L::get(qxy, i) + sigma * exy;
     const R qn = sqrt(abs2(bxx) + abs2(byy) + R(2) * abs2(bxy));
-    const R sq = R(1) / fmax(R(1), qn / a0);
+    const R sq = rcp(fmax(R(1), qn * ia0));
     const P nxx = bxx * sq, nyy = byy * sq, nxy = bxy * sq;
     L::put(qxx, i, nxx);
     L::put(qyy, i, nyy);
@@ -529,7 +535,7 @@ __device__ __forceinline__ void dual_pixel(const SliceDev& d, int x, int y, type
   const P ax = L::get(px, i) + sigma * tx;
   const P ay = L::get(py, i) + sigma * ty;
   const R pn = sqrt(abs2(ax) + abs2(ay));
-  const R sp = TGV ? R(1) / fmax(R(1), pn / a1) : R(1) / fmax(R(1), pn);
+  const R sp = rcp(fmax(R(1), TGV ? pn * ia1 : pn));
   const P nx = ax * sp, ny = ay * sp;
   L::put(px, i, nx);
   L::put(py, i, ny);
@@ -631,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_tv_primal(const SliceDev* sl, const 
     const P uo = u[i];
     const P div = dmx<P>(px, x, i, W) + dmy<P>(py, y, i, W, H);
     P un = uo + tau * (div + lambda * f[i]);
-    un = un / (R(1) + tau * lambda);
+    un = un * rcp(R(1) + tau * lambda);
     u[i] = un;
     ub[i] = (R(1) + theta) * un - theta * uo;
     if (GAP) {
@@ -668,7 +674,7 @@ __global__ __launch_bounds__(kBlock) void k_tgv_primal(const SliceDev* sl, const
     const P uo = u[i];
     const P div = dmx<P>(px, x, i, W) + dmy<P>(py, y, i, W, H);
     P un = uo + tau * (div + lambda * f[i]);
-    un = un / (R(1) + tau * lambda);
+    un = un * rcp(R(1) + tau * lambda);
     u[i] = un;
     ub[i] = (R(1) + theta) * un - theta * uo;
     vstep_pixel<P, S>(d, x, y, tau, theta);
@@ -693,7 +699,7 @@ __global__ __launch_bounds__(kBlock) void k_tgv_dual(const SliceDev* sl, const i
   for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
     const int yy = idx / W;
     const int x = idx - yy * W, y = ya + yy;
-    dual_pixel<P, S, true, GAP>(d, x, y, sigma, a0, a1, t);
+    dual_pixel<P, S, true, GAP>(d, x, y, sigma, rcp(a0), rcp(a1), t);
     if (GAP) {
       const size_t i = size_t(y) * W + x;
       t[3] += dabs2(ub[i] - f[i]);
@@ -845,7 +851,8 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
     const R sigma = R(d.sigma);
     const R a0 = sched_alpha<R>(d, d.alpha0, d.alpha0_end, k);
     const R a1 = sched_alpha<R>(d, d.alpha1, d.alpha1_end, k);
-    for (int x = threadIdx.x; x < W; x += NT) dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, a0, a1, t);
+    const R ia0 = rcp(a0), ia1 = rcp(a1);
+    for (int x = threadIdx.x; x < W; x += NT) dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, ia0, ia1, t);
   }
   if (pf) {
 #pragma unroll
@@ -898,7 +905,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   // coil c are contiguous in the compact layout.  With a compile-time length
   // they are loaded into registers here, together with the T1 gather, so that
   // their latency is not paid again after the forward FFT.
-  const R sigma = R(d.sigma), lambda = R(d.lambda);
+  const R sigma = R(d.sigma), lambda = R(d.lambda), inv_sl = rcp(R(1) + sigma * lambda);
   const C* fk = static_cast<const C*>(d.fk) + (size_t(c) * nc + ks0) * Hk;
   C* rk = static_cast<C*>(d.rk) + (size_t(c) * nc + ks0) * Hk;
   const R* mk = Mc + size_t(ks0) * Hk;
@@ -942,7 +949,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
       const int cl = divn<NF>(e, Hk, rHk), kfe = e - cl * Hk;
       const C a = (m * scale) * res[cl * Hk + fshift_src(kfe, Hk)];
       const C tmp = a - f;
-      const C rn = (r + sigma * tmp) / (R(1) + sigma * lambda);
+      const C rn = (r + sigma * tmp) * inv_sl;
       rk[e] = rn;
       if (GAP) {
         t[0] += dabs2(tmp);
