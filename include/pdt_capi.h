@@ -188,9 +188,10 @@ typedef struct pdt_kernel_prof {
 } pdt_kernel_prof;
 PDT_API int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, int cap, int* n);
 
-/* NEW: autotuner.  Times every candidate tile shape (coils per LDS chunk and
- * columns per group of the MRI passes, rows per work item of the denoising
- * passes) on the configured solvers (one batch) with hipEvents, keeps the
+/* NEW: autotuner.  Times every candidate tile shape (coils per LDS chunk of
+ * the forward and of the adjoint row pass, columns per group of the column
+ * pass, rows per work item of the denoising passes) on the configured solvers
+ * (one batch) with hipEvents, keeps the
  * fastest in the process-wide tile cache (persisted to the file named by the
  * environment variable PDT_TUNE_CACHE, if set) and returns it.  Advances the
  * solver state: call before the solves and reset with set_input0. */

@@ -458,6 +458,7 @@ struct pdt_op {
     SliceDev d = geometry();
     d.ngroups = (ncols + cg - 1) / cg;
     d.ngr = (C + cb - 1) / cb;
+    d.ngr_a = d.ngr;
     d.T1 = T1.p;
     d.T2 = T2.p;
     d.wpart = wpart.p;
@@ -481,8 +482,13 @@ struct pdt_op {
     L.lds_cols = lds_cols;
     L.fft_w = W;
     L.fft_h = Hk;
-    L.split_primal = d.ngr > 1;
+    L.split_primal = d.ngr_a > 1;
     L.primal_threads = primal_block(W);
+    L.rowgroups_a = L.rowgroups;
+    L.nrg_a = L.nrg;
+    L.cb_a = L.cb;
+    L.lds_rows_a = L.lds_rows;
+    L.adj_threads = kRowThreads;
     const int dt = dtype == PDT_C128 ? DT_C128 : DT_C64;
     if (forward) {
       run_zero(st, dst, size_t(W) * Hk * C * esize(cplx_of(dtype)));
@@ -672,16 +678,16 @@ void prepare_op_data(pdt_solver* s) {
 struct Engine {
   int kind = 0, dtype = 0;
   bool fp16 = false;
-  int force_cb = 0, force_cg = 0, force_rpi = 0;  // autotuner overrides (0: cache / default)
+  int force_cb = 0, force_cg = 0, force_rpi = 0, force_cb_a = 0;  // autotuner overrides (0: cache / default)
   bool plan_only = false;                         // build(): stop after the signature
   std::string sig;
   TuneKey key{};
   std::vector<pdt_solver*> sv;
   hipStream_t stream = nullptr;
-  DevBuf d_slices, d_rows, d_rg, d_cols, d_part, d_ctl, d_stats;
+  DevBuf d_slices, d_rows, d_rg, d_rg_a, d_cols, d_part, d_ctl, d_stats;
   int* h_ctl = nullptr;       // pinned mirrors
   double* h_stats = nullptr;
-  int nrows = 0, nrg = 0, ncolitems = 0;
+  int nrows = 0, nrg = 0, nrg_a = 0, ncolitems = 0;
   Launch L;                             // every slice: gap reduction, full-size buffers
   std::vector<Launch> G;                // one launch per group of slices with equal FFT lengths
   std::vector<std::vector<int>> groups;  // slice indices of each group
@@ -709,8 +715,8 @@ struct Engine {
     const int ns = (int)sv.size();
     const bool mri = s0->is_op();
     // tile choices (shared by the whole batch)
-    int cb = 64, cg = 4, rpi = 1;
-    size_t lr = 0, lc = 0;
+    int cb = 64, cg = 4, rpi = 1, cb_a = 64;
+    size_t lr = 0, lc = 0, lra = 0;
     int maxW = 0, maxHk = 0;
     for (auto* s : sv) {
       maxW = std::max(maxW, s->W);
@@ -728,9 +734,12 @@ struct Engine {
       TuneVal tv = tune_lookup(key);
       cb = force_cb > 0 ? force_cb : tv.cb > 0 ? tv.cb : kDefaultCb;
       cg = force_cg > 0 ? force_cg : tv.cg > 0 ? tv.cg : kDefaultCg;
+      cb_a = force_cb_a > 0 ? force_cb_a : tv.cb_a > 0 ? tv.cb_a : cb;
       cb = std::max(1, std::min(cb, max_cb(dt, maxW, maxNc, maxC)));
+      cb_a = std::max(1, std::min(cb_a, max_cb(dt, maxW, maxNc, maxC)));
       cg = std::max(1, std::min(cg, max_cg(dt, maxHk, maxNc)));
       lr = lds_rows_bytes(dt, cb, maxW, maxNc);
+      lra = lds_rows_bytes(dt, cb_a, maxW, maxNc);
       lc = lds_cols_bytes(dt, cg, maxHk);
     } else {
       key = TuneKey{kind, dt, maxW, sv[0]->H, 0, 0, 0, ns};
@@ -797,7 +806,8 @@ struct Engine {
         d.T2 = s->T2.p;
         d.ngroups = (d.ncols + cg - 1) / cg;
         d.ngr = (d.C + cb - 1) / cb;
-        s->wpart.ensure(size_t(d.ngr) * d.W * d.H * s->pe());
+        d.ngr_a = (d.C + cb_a - 1) / cb_a;
+        s->wpart.ensure(size_t(d.ngr_a) * d.W * d.H * s->pe());
         d.wpart = s->wpart.p;
       }
       hs[i] = d;
@@ -821,16 +831,18 @@ struct Engine {
     L.rpi = rpi;
     L.cb = cb;
     L.cg = cg;
-    std::vector<int2> rows, rgs, cols;
+    L.cb_a = cb_a;
+    std::vector<int2> rows, rgs, rgs_a, cols;
     std::vector<GroupRange> gr;
-    make_lists(std::vector<char>(ns, 1), rows, rgs, cols, gr);
+    make_lists(std::vector<char>(ns, 1), rows, rgs, rgs_a, cols, gr);
     nrows = (int)rows.size();
     nrg = (int)rgs.size();
+    nrg_a = (int)rgs_a.size();
     ncolitems = (int)cols.size();
     // signature of everything the kernels and graphs depend on (the control
     // and statistics words are owned by the engine and added below)
     sig.assign(reinterpret_cast<const char*>(hs.data()), hs.size() * sizeof(SliceDev));
-    const int tiles[4] = {cb, cg, rpi, int(sv[0]->prm.gap_every)};
+    const int tiles[5] = {cb, cg, rpi, int(sv[0]->prm.gap_every), cb_a};
     sig.append(reinterpret_cast<const char*>(tiles), sizeof(tiles));
     if (plan_only) return;
     d_ctl.alloc(size_t(ns) * CTL_WORDS * sizeof(int));
@@ -844,8 +856,9 @@ struct Engine {
     d_slices.alloc(hs.size() * sizeof(SliceDev));
     d_rows.alloc(std::max<size_t>(1, rows.size()) * sizeof(int2));
     d_rg.alloc(std::max<size_t>(1, rgs.size()) * sizeof(int2));
+    d_rg_a.alloc(std::max<size_t>(1, rgs_a.size()) * sizeof(int2));
     d_cols.alloc(std::max<size_t>(1, cols.size()) * sizeof(int2));
-    const size_t nitem = std::max<size_t>(1, std::max({rows.size(), rgs.size(), cols.size()}));
+    const size_t nitem = std::max<size_t>(1, std::max({rows.size(), rgs.size(), rgs_a.size(), cols.size()}));
     d_part.alloc(4 * nitem * kGapTerms * sizeof(double));
     L.stream = stream;
     L.sl = d_slices.as<SliceDev>();
@@ -854,31 +867,35 @@ struct Engine {
     L.nrows = nrows;
     L.rowgroups = d_rg.as<int2>();
     L.nrg = nrg;
+    L.rowgroups_a = d_rg_a.as<int2>();
+    L.nrg_a = nrg_a;
     L.cols = d_cols.as<int2>();
     L.ncolitems = ncolitems;
     for (int k = 0; k < 4; ++k) L.part[k] = d_part.as<double>() + size_t(k) * nitem * kGapTerms;
     L.gap_rows = L.part[2];
     L.lds_rows = lr;
+    L.lds_rows_a = lra;
     L.lds_cols = lc;
-    upload_lists(rows, rgs, cols, gr);
+    upload_lists(rows, rgs, rgs_a, cols, gr);
   }
 
   // Item ranges of one slice group in the work lists.
   struct GroupRange {
     int g;  // index into groups
-    int row0, nrow, rg0, nrg, col0, ncol;
+    int row0, nrow, rg0, nrg, rga0, nrga, col0, ncol;
   };
 
   // Work lists of the slices with use[i] != 0, group after group: row items
-  // (slice, first row), row-group items (slice, row * ngr + coil group) and
+  // (slice, first row), row-group items of the forward (slice, row * ngr +
+  // coil group) and adjoint (slice, row * ngr_a + coil group) row passes and
   // column items (slice, coil * ngroups + column group).  Sets the item
   // ranges of hs; slices left out get empty ranges.
   void make_lists(const std::vector<char>& use, std::vector<int2>& rows, std::vector<int2>& rgs,
-                  std::vector<int2>& cols, std::vector<GroupRange>& gr) {
+                  std::vector<int2>& rgs_a, std::vector<int2>& cols, std::vector<GroupRange>& gr) {
     const bool mri = sv[0]->is_op();
     const int step = mri ? 1 : L.rpi;
     for (size_t g = 0; g < groups.size(); ++g) {
-      GroupRange r{int(g), (int)rows.size(), 0, (int)rgs.size(), 0, (int)cols.size(), 0};
+      GroupRange r{int(g), (int)rows.size(), 0, (int)rgs.size(), 0, (int)rgs_a.size(), 0, (int)cols.size(), 0};
       for (int i : groups[g]) {
         SliceDev& d = hs[i];
         d.row_beg = d.row_end = d.rg_beg = d.rg_end = d.col_beg = d.col_end = 0;
@@ -891,6 +908,9 @@ struct Engine {
           for (int y = 0; y < d.H; ++y)
             for (int q = 0; q < d.ngr; ++q) rgs.push_back(make_int2(i, y * d.ngr + q));
         d.rg_end = (int)rgs.size();
+        if (mri)
+          for (int y = 0; y < d.H; ++y)
+            for (int q = 0; q < d.ngr_a; ++q) rgs_a.push_back(make_int2(i, y * d.ngr_a + q));
         d.col_beg = (int)cols.size();
         if (mri)
           for (int c = 0; c < d.C; ++c)
@@ -899,6 +919,7 @@ struct Engine {
       }
       r.nrow = (int)rows.size() - r.row0;
       r.nrg = (int)rgs.size() - r.rg0;
+      r.nrga = (int)rgs_a.size() - r.rga0;
       r.ncol = (int)cols.size() - r.col0;
       if (r.nrow > 0) gr.push_back(r);
     }
@@ -907,11 +928,12 @@ struct Engine {
   // Copies slices and lists to the device (the stream is idle) and derives
   // one launch per non-empty group: its sub-ranges of the lists and of the
   // partial-sum buffers, its FFT lengths and LDS sizes.
-  void upload_lists(const std::vector<int2>& rows, const std::vector<int2>& rgs, const std::vector<int2>& cols,
-                    const std::vector<GroupRange>& gr) {
+  void upload_lists(const std::vector<int2>& rows, const std::vector<int2>& rgs, const std::vector<int2>& rgs_a,
+                    const std::vector<int2>& cols, const std::vector<GroupRange>& gr) {
     HIPC(hipMemcpy(d_slices.p, hs.data(), hs.size() * sizeof(SliceDev), hipMemcpyHostToDevice));
     if (!rows.empty()) HIPC(hipMemcpy(d_rows.p, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
     if (!rgs.empty()) HIPC(hipMemcpy(d_rg.p, rgs.data(), rgs.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (!rgs_a.empty()) HIPC(hipMemcpy(d_rg_a.p, rgs_a.data(), rgs_a.size() * sizeof(int2), hipMemcpyHostToDevice));
     if (!cols.empty()) HIPC(hipMemcpy(d_cols.p, cols.data(), cols.size() * sizeof(int2), hipMemcpyHostToDevice));
     const bool mri = sv[0]->is_op();
     const int dt = dtk();
@@ -923,6 +945,8 @@ struct Engine {
       x.nrows = r.nrow;
       x.rowgroups = L.rowgroups + r.rg0;
       x.nrg = r.nrg;
+      x.rowgroups_a = L.rowgroups_a + r.rga0;
+      x.nrg_a = r.nrga;
       x.cols = L.cols + r.col0;
       x.ncolitems = r.ncol;
       if (mri) {
@@ -937,11 +961,14 @@ struct Engine {
         x.fft_h = d0.Hk;
         x.primal_threads = primal_block(d0.W);
         x.split_primal = false;
-        for (int i : groups[r.g]) x.split_primal = x.split_primal || hs[i].ngr > 1;
+        for (int i : groups[r.g]) x.split_primal = x.split_primal || hs[i].ngr_a > 1;
         x.lds_rows = lds_rows_bytes(dt, x.cb, d0.W, nc);
+        x.lds_rows_a = lds_rows_bytes(dt, x.cb_a, d0.W, nc);
         x.lds_cols = lds_cols_bytes(dt, x.cg, d0.Hk);
+        // wide blocks for the adjoint pass when an item holds many coil rows
+        x.adj_threads = x.cb_a * d0.W >= 8 * kRowThreads ? kRowAdjThreadsWide : kRowThreads;
       }
-      listed_items += r.nrow + r.nrg + r.ncol;
+      listed_items += r.nrow + r.nrg + r.nrga + r.ncol;
       G.push_back(x);
     }
   }
@@ -959,14 +986,14 @@ struct Engine {
       if (use[i]) {
         const SliceDev& d = hs[i];
         const int step = sv[0]->is_op() ? 1 : L.rpi;
-        items += (d.H + step - 1) / step + (sv[0]->is_op() ? d.H * d.ngr + d.C * d.ngroups : 0);
+        items += (d.H + step - 1) / step + (sv[0]->is_op() ? d.H * (d.ngr + d.ngr_a) + d.C * d.ngroups : 0);
       }
     }
     if (items == 0 || 6 * items > 5 * listed_items) return;
-    std::vector<int2> rows, rgs, cols;
+    std::vector<int2> rows, rgs, rgs_a, cols;
     std::vector<GroupRange> gr;
-    make_lists(use, rows, rgs, cols, gr);
-    upload_lists(rows, rgs, cols, gr);
+    make_lists(use, rows, rgs, rgs_a, cols, gr);
+    upload_lists(rows, rgs, rgs_a, cols, gr);
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();
   }
@@ -995,12 +1022,12 @@ struct Engine {
         const double S = s->op->kind == PDT_OP_SAMPLING ? 0.0 : C * N * pe;  // sensitivities on the image window
         const double T = C * H * nc * pe;                                   // T1 or T2
         const bool tgv = kind != PDT_SOLVER_TV_OP;
-        const double Wp = double(d.ngr) * N * pe;  // per-coil-group partial adjoints
+        const double Wp = double(d.ngr_a) * N * pe;  // per-coil-group partial adjoints
         const double Pr = (tgv ? 14 * pe : 5 * pe) * N;  // primal step
         if (stage == 0) tot += (tgv ? 13 * pe : 5 * pe) * N + S + T;
         else if (stage == 1) tot += 2 * T + 3 * C * nc * Hk * pe + nc * Hk * (pe / 2);
-        else if (stage == 2) tot += T + S + (d.ngr > 1 ? Wp : Pr);  // one coil group: primal step in the same kernel
-        else tot += d.ngr > 1 ? Wp + Pr : 0.0;
+        else if (stage == 2) tot += T + S + (d.ngr_a > 1 ? Wp : Pr);  // one coil group: primal step in the same kernel
+        else tot += d.ngr_a > 1 ? Wp + Pr : 0.0;
       }
     }
     return tot;
@@ -1681,7 +1708,10 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* ou
     std::vector<pdt_solver*> sv(solvers, solvers + count);
     for (auto* s : sv) prepare_solve(s, 0);
     const bool mri = sv[0]->is_op();
-    std::vector<std::array<int, 3>> cand;
+    // candidates: coils per forward row item, columns per column item, rows
+    // per denoising item, coils per adjoint row item (the forward choice, or
+    // every coil of the row, which also fuses the u step into that pass)
+    std::vector<std::array<int, 4>> cand;
     if (mri) {
       int maxC = 1;
       for (auto* s : sv) maxC = std::max(maxC, s->op->C);
@@ -1690,12 +1720,15 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* ou
       cbs.erase(std::unique(cbs.begin(), cbs.end()), cbs.end());
       for (int cb : cbs)
         if (cb <= maxC)
-          for (int cg : {1, 2, 4}) cand.push_back({cb, cg, 0});
+          for (int cg : {1, 2, 4}) {
+            cand.push_back({cb, cg, 0, cb});
+            if (cb != maxC) cand.push_back({cb, cg, 0, maxC});
+          }
     } else {
-      for (int rpi : {1, 2, 4, 8}) cand.push_back({0, 0, rpi});
+      for (int rpi : {1, 2, 4, 8}) cand.push_back({0, 0, rpi, 0});
     }
     double best = 1e300;
-    std::array<int, 3> bc{0, 0, 0};
+    std::array<int, 4> bc{0, 0, 0, 0};
     TuneKey key{};
     for (const auto& c : cand) {
       Engine e;
@@ -1703,16 +1736,17 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* ou
       e.force_cb = c[0];
       e.force_cg = c[1];
       e.force_rpi = c[2];
+      e.force_cb_a = c[3];
       e.build();
       e.reset_ctl();
       const double ms = time_iteration(e, reps);
       key = e.key;
       if (ms < best) {
         best = ms;
-        bc = {e.L.cb, e.L.cg, e.L.rpi};
+        bc = {e.L.cb, e.L.cg, e.L.rpi, e.L.cb_a};
       }
     }
-    tune_store(key, TuneVal{mri ? bc[0] : 0, mri ? bc[1] : 0, mri ? 0 : bc[2]});
+    tune_store(key, mri ? TuneVal{bc[0], bc[1], 0, bc[3]} : TuneVal{0, 0, bc[2], 0});
     if (out_cb) *out_cb = bc[0];
     if (out_cg) *out_cg = bc[1];
     if (out_rpi) *out_rpi = bc[2];

@@ -13,6 +13,7 @@ namespace pdt {
 
 constexpr int kBlock = 256;       // threads per workgroup of the stencil kernels (4 wave64)
 constexpr int kRowThreads = 256;  // threads per workgroup of the MRI row FFT passes
+constexpr int kRowAdjThreadsWide = 1024;  // adjoint row pass with many coils per item
 constexpr int kColThreads = 256;  // threads per workgroup of the MRI column pass
 constexpr int kPrefetch = 8;       // row-pass elements per thread held in registers ahead of use
 constexpr int kColPrefetchCols = 4;  // column groups up to this width prefetch their prox operands
@@ -51,7 +52,8 @@ struct SliceDev {
   int Hk, y0, C;  // k-space FE length, FE offset of the image window, coils
   int ncols;      // listed (sampled) PE columns
   int ngroups;    // column groups of the column pass (per coil)
-  int ngr;        // coil groups of the row passes
+  int ngr;        // coil groups of the forward row pass
+  int ngr_a;      // coil groups of the adjoint row pass (planes of wpart; 1: u step fused)
   int kind;
   // scalar parameters
   double tau, sigma, theta, lambda;

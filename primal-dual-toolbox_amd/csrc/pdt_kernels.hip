@@ -1012,9 +1012,9 @@ __device__ __forceinline__ void mri_primal_pixel(const SliceDev& d, int x, int y
   }
 }
 
-// Row pass, adjoint.  Item = (slice, y * ngr + g): for each coil of group g,
+// Row pass, adjoint.  Item = (slice, y * ngr_a + g): for each coil of group g,
 // T2 row -> ifftshift -> inverse PE-FFT -> fftshift -> * conj(S_c), summed
-// over the group's coils in order.  With several coil groups (ngr > 1) the
+// over the group's coils in order.  With several coil groups (ngr_a > 1) the
 // sum goes to wpart[g][y][x] and k_mri_primal completes the step; with one
 // group the block holds all of (A^* r)(., y) and performs the primal step of
 // row y itself (MODE M_OP: writes A^* r to op_out).
@@ -1028,7 +1028,7 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   if (!slice_active(d)) return;
   using C = Cx<R>;
   const int W = NF > 0 ? NF : d.W, H = d.H, nc = d.ncols;
-  const int y = it.y / d.ngr, g = it.y - y * d.ngr;
+  const int y = it.y / d.ngr_a, g = it.y - y * d.ngr_a;
   const int c0 = g * cb, ncb = min(cb, d.C - c0);
   RowLds<C> L(smem_raw, W, cb);
   const C* gtw = static_cast<const C*>(d.planW.tw);
@@ -1080,7 +1080,7 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
     }
   }
   __syncthreads();
-  if (d.ngr > 1) {
+  if (d.ngr_a > 1) {
     C* wp = static_cast<C*>(d.wpart) + (size_t(g) * H + y) * W;
     for (int x = threadIdx.x; x < W; x += NT) {
       C acc = other[x];
@@ -1117,9 +1117,9 @@ __global__ __launch_bounds__(1024) void k_mri_primal(const SliceDev* sl, const i
   const int li = xcd_item(blockIdx.x, nitems);
   const int2 it = items[li];
   const SliceDev& d = sl[it.x];
-  if (!slice_active(d) || d.ngr == 1) return;
+  if (!slice_active(d) || d.ngr_a == 1) return;
   using C = Cx<R>;
-  const int W = d.W, H = d.H, ngr = d.ngr;
+  const int W = d.W, H = d.H, ngr = d.ngr_a;
   const size_t n = size_t(W) * H;
   const int ya = it.y, yb = min(H, ya + rpi);
   const C* wp = static_cast<const C*>(d.wpart);
@@ -1356,6 +1356,20 @@ static void with_fft_len(int n, Fn&& fn) {
   fn(std::integral_constant<int, 0>{});
 }
 
+// Adjoint row pass with L.adj_threads threads per block (kRowThreads, or
+// kRowAdjThreadsWide for items holding many coils).
+template <class R, int MODE, int NF>
+static void launch_rows_adj(const Launch& L, bool gap) {
+  constexpr int NR = kRowThreads, NW = kRowAdjThreadsWide;
+  if (L.adj_threads == NW) {
+    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NW, NF>), L.nrg_a, NW, L.lds_rows_a, L.stream, L.sl, L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
+    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NW, NF>), L.nrg_a, NW, L.lds_rows_a, L.stream, L.sl, L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
+  } else {
+    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NR, NF>), L.nrg_a, NR, L.lds_rows_a, L.stream, L.sl, L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
+    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NR, NF>), L.nrg_a, NR, L.lds_rows_a, L.stream, L.sl, L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
+  }
+}
+
 // MRI stages: 0 rows_fwd, 1 cols, 2 rows_adj, 3 primal.  The FFT kernels are
 // picked by L.fft_w (row length) and L.fft_h (column length).
 template <class R, int MODE>
@@ -1377,8 +1391,7 @@ static void mri_stage_m(const Launch& L, bool gap, int stage) {
   } else if (stage == 2) {
     with_fft_len<R>(L.fft_w, [&](auto nf) {
       constexpr int NF = decltype(nf)::value;
-      if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.gap_rows);
-      else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.gap_rows);
+      launch_rows_adj<R, MODE, NF>(L, gap);
     });
   } else {
     if (!L.split_primal) return;
@@ -1439,14 +1452,14 @@ static void op_forward(const Launch& L) {
 
 template <class R>
 static void op_adjoint(const Launch& L) {
-  constexpr int NR = kRowThreads, NC = kColThreads;
+  constexpr int NC = kColThreads;
   with_fft_len<R>(L.fft_h, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;
     PDT_LAUNCH_NT((k_cols<R, CM_ADJ, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
   });
   with_fft_len<R>(L.fft_w, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;
-    PDT_LAUNCH_NT((k_rows_adj<R, M_OP, false, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.gap_rows);
+    launch_rows_adj<R, M_OP, NF>(L, false);
   });
   if (L.split_primal)
     PDT_LAUNCH_NT((k_mri_primal<R, M_OP, false>), L.nrows, L.primal_threads, 0, L.stream, L.sl, L.rows, L.nrows, L.rpi, L.part[2]);
@@ -1472,6 +1485,11 @@ static bool set_lds_attrs(int b) {
     set((const void*)&k_rows_adj<R, M_TV, true, NR, NF>);
     set((const void*)&k_rows_adj<R, M_TGV, false, NR, NF>);
     set((const void*)&k_rows_adj<R, M_TGV, true, NR, NF>);
+    set((const void*)&k_rows_adj<R, M_OP, false, kRowAdjThreadsWide, NF>);
+    set((const void*)&k_rows_adj<R, M_TV, false, kRowAdjThreadsWide, NF>);
+    set((const void*)&k_rows_adj<R, M_TV, true, kRowAdjThreadsWide, NF>);
+    set((const void*)&k_rows_adj<R, M_TGV, false, kRowAdjThreadsWide, NF>);
+    set((const void*)&k_rows_adj<R, M_TGV, true, kRowAdjThreadsWide, NF>);
     set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF>);
     set((const void*)&k_cols<R, CM_SOLVE, true, NC, NF>);
     set((const void*)&k_cols<R, CM_FWD, false, NC, NF>);

@@ -31,6 +31,13 @@ struct Launch {
   double* gap_rows = nullptr;    // row-item partial sums indexed by global row item (primal in k_rows_adj)
   bool split_primal = true;      // some slice has several coil groups: k_mri_primal completes the step
   int primal_threads = 256;      // block size of k_mri_primal: rpi * W rounded up to 64, at most 1024
+  // adjoint row pass: its own coil groups (slice, row * ngr_a + group), LDS
+  // bytes and block size (kRowThreads or kRowAdjThreadsWide)
+  const int2* rowgroups_a = nullptr;
+  int nrg_a = 0;
+  int cb_a = 1;
+  size_t lds_rows_a = 0;
+  int adj_threads = 256;
 };
 
 // Block size of k_mri_primal covering rows of width W in one pass.

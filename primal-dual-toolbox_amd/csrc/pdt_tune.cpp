@@ -28,7 +28,8 @@ bool g_loaded = false;
 
 const char* cache_path() { return std::getenv("PDT_TUNE_CACHE"); }
 
-// File format: one entry per line, "kind dt W H Hk C nc ns : cb cg rpi".
+// File format: one entry per line, "kind dt W H Hk C nc ns : cb cg rpi [cb_a]"
+// (cb_a absent: the adjoint row pass uses cb).
 void load_locked() {
   if (g_loaded) return;
   g_loaded = true;
@@ -42,8 +43,10 @@ void load_locked() {
     TuneVal v{};
     char colon = 0;
     if (ss >> k.kind >> k.dt >> k.W >> k.H >> k.Hk >> k.C >> k.nc >> k.ns >> colon >> v.cb >> v.cg >> v.rpi &&
-        colon == ':')
+        colon == ':') {
+      if (!(ss >> v.cb_a)) v.cb_a = 0;
       g_cache[k] = v;
+    }
   }
 }
 
@@ -52,7 +55,8 @@ void save_locked() {
   if (!p) return;
   std::ofstream out(p, std::ios::trunc);
   for (const auto& kv : g_cache)
-    out << kv.first.str() << " : " << kv.second.cb << ' ' << kv.second.cg << ' ' << kv.second.rpi << '\n';
+    out << kv.first.str() << " : " << kv.second.cb << ' ' << kv.second.cg << ' ' << kv.second.rpi << ' '
+        << kv.second.cb_a << '\n';
 }
 }  // namespace
 

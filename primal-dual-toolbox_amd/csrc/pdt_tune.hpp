@@ -1,8 +1,9 @@
 // pdt_tune.hpp -- tile-shape cache of the autotuner.
 //
 // Tiles are keyed by the problem shape; values are the coils per LDS chunk
-// (cb) of the row kernels, the columns per group (cg) of the column kernel and
-// the rows per item (rpi) of the denoising kernels.  A value of 0 means "use
+// of the forward (cb) and adjoint (cb_a) row kernels, the columns per group
+// (cg) of the column kernel and the rows per item (rpi) of the denoising
+// kernels.  A value of 0 means "use
 // the built-in default".  The cache is process-wide and, when the environment
 // variable PDT_TUNE_CACHE names a file, loaded from / saved to that file, so a
 // tuned choice is deterministic across runs.
@@ -19,7 +20,7 @@ struct TuneKey {
 };
 
 struct TuneVal {
-  int cb = 0, cg = 0, rpi = 0;
+  int cb = 0, cg = 0, rpi = 0, cb_a = 0;
 };
 
 TuneVal tune_lookup(const TuneKey& k);
