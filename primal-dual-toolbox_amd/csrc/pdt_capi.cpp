@@ -963,10 +963,10 @@ struct Engine {
         x.split_primal = false;
         for (int i : groups[r.g]) x.split_primal = x.split_primal || hs[i].ngr_a > 1;
         x.lds_rows = lds_rows_bytes(dt, x.cb, d0.W, nc);
-        x.lds_rows_a = lds_rows_bytes(dt, x.cb_a, d0.W, nc);
         x.lds_cols = lds_cols_bytes(dt, x.cg, d0.Hk);
         // wide blocks for the adjoint pass when an item holds many coil rows
         x.adj_threads = x.cb_a * d0.W >= 8 * kRowThreads ? kRowAdjThreadsWide : kRowThreads;
+        x.lds_rows_a = lds_rows_bytes(dt, x.cb_a, d0.W, nc, x.adj_threads != kRowAdjThreadsWide);
       }
       listed_items += r.nrow + r.nrg + r.nrga + r.ncol;
       G.push_back(x);

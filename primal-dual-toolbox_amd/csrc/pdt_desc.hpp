@@ -21,6 +21,9 @@ constexpr int kMaxStages = 14;    // FFT stages per transform
 constexpr int kGapTerms = 8;      // partial-sum slots per work item
 constexpr int kMaxRowLen = 4096;  // W limit of the MRI row kernels
 constexpr size_t kLdsMax = 160 * 1024;  // LDS per workgroup on gfx950
+// LDS elements of an FFT buffer of n entries: one pad slot per 8 entries
+// (the compile-time-planned kernels address element e at e + e / 8).
+constexpr int lds_pad(int n) { return n + n / 8 + 1; }
 
 // Solver kinds (match PDT_SOLVER_* of pdt_capi.h).
 enum Kind { K_TV = 0, K_TGV = 1, K_TGV_MRI = 2, K_TV_OP = 3, K_TGV_OP = 4 };

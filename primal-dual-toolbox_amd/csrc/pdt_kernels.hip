@@ -279,9 +279,11 @@ __device__ void butterfly_gen(const Cx<R>* in, Cx<R>* out, int r, int nr, int p,
 // loop over output pairs stays rolled: only x_0, s and d are live across it,
 // which keeps this rarely used radix from raising the register count of the
 // kernels that contain it.  h^2 complex-by-real products per output pair.
+// in[j * nr] are the inputs, out[q * p] the outputs; tw[m * tstep] =
+// e^(-2 pi i m / P) (tstep = n / P of the length-n table).
 template <class R, int P>
 __device__ __forceinline__ void butterfly_odd(const Cx<R>* in, Cx<R>* out, int nr, int p, const Cx<R>* tw,
-                                              int ts, bool inv) {
+                                              int ts, bool inv, int tstep) {
   constexpr int h = (P - 1) / 2;
   const Cx<R> x0 = in[0];
   Cx<R> s[h], d[h];
@@ -304,7 +306,7 @@ __device__ __forceinline__ void butterfly_odd(const Cx<R>* in, Cx<R>* out, int n
     for (int m = 1; m <= h; ++m) {
       jk += k;
       if (jk >= P) jk -= P;
-      const Cx<R> w = tw[jk * nr];
+      const Cx<R> w = tw[jk * tstep];
       A = A + s[m - 1] * w.re;
       B = B - d[m - 1] * w.im;
     }
@@ -327,6 +329,7 @@ __device__ __forceinline__ int divn(int a, int n, float rn) {
   if constexpr (NF > 0) return a / NF;
   else return qdiv(a, n, rn);
 }
+
 
 // Transforms nseq sequences of length pl.n stored back to back in a; b is
 // scratch of the same size; tw is the twiddle table (LDS or global).
@@ -395,11 +398,35 @@ constexpr int ct_nstages(int n) {
   while (ct_radix(n, s) != 1) ++s;
   return s;
 }
+// The padded LDS layout (element e at e + e / 8) keeps every input and
+// output of a butterfly at a constant padded stride when n, every n / r and
+// every earlier-radix product p (or r, when p = 1) are multiples of 8.
+constexpr bool ct_pad_ok(int n) {
+  if (n % 8 != 0) return false;
+  int p = 1;
+  for (int s = 0; s < ct_nstages(n); ++s) {
+    const int r = ct_radix(n, s);
+    if ((n / r) % 8 != 0 || (p == 1 ? r != 8 : p % 8 != 0)) return false;
+    p *= r;
+  }
+  return true;
+}
+
+// LDS slot of element e of an FFT buffer.  Kernels with a compile-time length
+// (NF > 0) pad one element after every 8: the Stockham stages write with
+// strides of 8 elements (and multiples), which would otherwise put the 64
+// lanes of a wave on a few LDS banks.
+template <int NF, bool LP = true>
+__device__ __forceinline__ int lp(int e) {
+  if constexpr (LP && NF > 0 && ct_pad_ok(NF)) return e + (e >> 3);
+  else return e;
+}
 
 // TW: stage twiddles present (false for the first stage, where they are 1).
+// nr, p: LDS strides of the inputs and outputs; tstep = n / RR.
 template <class R, int RR, bool TW>
 __device__ __forceinline__ void butterfly_ct(const Cx<R>* in, Cx<R>* out, int ts, const Cx<R>* tw, bool inv,
-                                             int nr, int p) {
+                                             int nr, int p, int tstep) {
   if constexpr (RR == 2 || RR == 3 || RR == 4 || RR == 5 || RR == 8) {
     Cx<R> x[RR];
     x[0] = in[0];
@@ -410,25 +437,31 @@ __device__ __forceinline__ void butterfly_ct(const Cx<R>* in, Cx<R>* out, int ts
     for (int q = 0; q < RR; ++q) out[q * p] = x[q];
   } else {
     static_assert(RR % 2 == 1, "even radices are factored into 2, 4 and 8");
-    butterfly_odd<R, RR>(in, out, nr, p, tw, ts, inv);
+    butterfly_odd<R, RR>(in, out, nr, p, tw, ts, inv, tstep);
   }
 }
 
-template <class R, int N, int S, int P, int NT, bool INV>
+template <class R, int N, int S, int P, int NT, bool INV, bool LP>
 __device__ __forceinline__ Cx<R>* fft_ct_stage(Cx<R>* a, Cx<R>* b, int nseq, const Cx<R>* tw) {
   if constexpr (S == ct_nstages(N)) {
     return a;
   } else {
     constexpr int r = ct_radix(N, S), nr = N / r, tws = N / (P * r);
+    // padded layout (ct_pad_ok): every input and output of a butterfly
+    // sits at a constant padded stride from its first one
+    constexpr bool PAD = LP && ct_pad_ok(N);
+    constexpr int NP = PAD ? N + N / 8 : N, nrp = PAD ? nr + nr / 8 : nr, pp = PAD && P > 1 ? P + P / 8 : P;
     const int total = nseq * nr;
     for (int t = threadIdx.x; t < total; t += NT) {
       const int seq = t / nr;
       const int i = t - seq * nr;
       const int k = i % P;
-      butterfly_ct<R, r, (P > 1)>(a + seq * N + i, b + seq * N + (i - k) * r + k, k * tws, tw, INV, nr, P);
+      const int o = (i - k) * r + k;
+      const int ip = PAD ? i + (i >> 3) : i, op = PAD ? o + (o >> 3) : o;
+      butterfly_ct<R, r, (P > 1)>(a + seq * NP + ip, b + seq * NP + op, k * tws, tw, INV, nrp, pp, nr);
     }
     __syncthreads();
-    return fft_ct_stage<R, N, S + 1, P * r, NT, INV>(b, a, nseq, tw);
+    return fft_ct_stage<R, N, S + 1, P * r, NT, INV, LP>(b, a, nseq, tw);
   }
 }
 
@@ -436,9 +469,9 @@ __device__ __forceinline__ Cx<R>* fft_ct_stage(Cx<R>* a, Cx<R>* b, int nseq, con
 // runtime plan when NF == 0.  Each FFT kernel is instantiated per length, so
 // its register budget is that of its own plan only (the runtime plan with
 // its radix switch needs several times the registers of a fixed one).
-template <class R, int NT, bool INV, int NF>
+template <class R, int NT, bool INV, int NF, bool LP = true>
 __device__ __forceinline__ Cx<R>* fft_n(Cx<R>* a, Cx<R>* b, int nseq, const FftPlanD& pl, const Cx<R>* tw) {
-  if constexpr (NF > 0) return fft_ct_stage<R, NF, 0, 1, NT, INV>(a, b, nseq, tw);
+  if constexpr (NF > 0) return fft_ct_stage<R, NF, 0, 1, NT, INV, LP>(a, b, nseq, tw);
   else return lds_fft<R, NT>(a, b, nseq, pl, tw, INV);
 }
 
@@ -789,11 +822,12 @@ struct RowLds {
   C* a;
   C* b;
   int* cols;
-  __device__ RowLds(char* raw, int W, int cb) {
+  __device__ RowLds(char* raw, int W, int cb, bool padded = true) {
+    const int n = padded ? lds_pad(cb * W) : cb * W;
     tw = reinterpret_cast<C*>(raw);
     a = tw + W;
-    b = a + size_t(cb) * W;
-    cols = reinterpret_cast<int*>(b + size_t(cb) * W);
+    b = a + n;
+    cols = reinterpret_cast<int*>(b + n);
   }
 };
 
@@ -842,7 +876,7 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
       const int cl = divn<NF>(e, W, rW), x = e - cl * W;
       C z = src[x];
       if (S) z = z * Sg[cl * G + x];
-      L.a[cl * W + ishift_dst(x, W)] = z;
+      L.a[lp<NF>(cl * W + ishift_dst(x, W))] = z;
     }
   }
   double t[kGapTerms] = {0};
@@ -860,7 +894,7 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
       const int e = threadIdx.x + k * NT;
       if (e < ncb * W) {
         const int cl = divn<NF>(e, W, rW), x = e - cl * W;
-        L.a[cl * W + ishift_dst(x, W)] = S ? ps[k] * pS[k] : ps[k];
+        L.a[lp<NF>(cl * W + ishift_dst(x, W))] = S ? ps[k] * pS[k] : ps[k];
       }
     }
   }
@@ -870,7 +904,7 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
   const int cstride = H * nc;
   for (int e = threadIdx.x; e < ncb * nc; e += NT) {
     const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
-    T1[cl * cstride + ks] = res[cl * W + fshift_src(L.cols[ks], W)];
+    T1[cl * cstride + ks] = res[lp<NF>(cl * W + fshift_src(L.cols[ks], W))];
   }
   if (GAP) block_sum<3, NT>(reinterpret_cast<double(&)[3]>(t), part + size_t(li) * kGapTerms);
 }
@@ -895,7 +929,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   const R* Mc = static_cast<const R*>(d.Mc);
   C* tw = reinterpret_cast<C*>(smem_raw);
   C* bufA = tw + Hk;
-  C* bufB = bufA + cg * Hk;
+  C* bufB = bufA + lds_pad(cg * Hk);
   const C* gtw = static_cast<const C*>(d.planH.tw);
   for (int e = threadIdx.x; e < Hk; e += NT) tw[e] = gtw[e];
   const float rHk = 1.0f / float(Hk), rncg = 1.0f / float(ncg);
@@ -930,7 +964,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
       const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
       const int fe = ishift_src(j, Hk);
       const int yy = fe - y0;
-      bufA[cl * Hk + j] = (yy >= 0 && yy < H) ? T1[yy * nc + cl] : pzero<C>();
+      bufA[lp<NF>(cl * Hk + j)] = (yy >= 0 && yy < H) ? T1[yy * nc + cl] : pzero<C>();
     }
     __syncthreads();
     res = fft_n<R, NT, false, NF>(bufA, bufB, ncg, d.planH, tw);
@@ -940,14 +974,14 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
       for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
         const int cl = divn<NF>(e, Hk, rHk), kfe = e - cl * Hk;
         const R m = Mc[size_t(ks0 + cl) * Hk + kfe];
-        out[(size_t(c) * Hk + kfe) * W + d.cols[ks0 + cl]] = (m * scale) * res[cl * Hk + fshift_src(kfe, Hk)];
+        out[(size_t(c) * Hk + kfe) * W + d.cols[ks0 + cl]] = (m * scale) * res[lp<NF>(cl * Hk + fshift_src(kfe, Hk))];
       }
       return;
     }
     // prox of r on A ubar - f
     auto prox = [&](int e, R m, C f, C r) {
       const int cl = divn<NF>(e, Hk, rHk), kfe = e - cl * Hk;
-      const C a = (m * scale) * res[cl * Hk + fshift_src(kfe, Hk)];
+      const C a = (m * scale) * res[lp<NF>(cl * Hk + fshift_src(kfe, Hk))];
       const C tmp = a - f;
       const C rn = (r + sigma * tmp) * inv_sl;
       rk[e] = rn;
@@ -955,7 +989,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
         t[0] += dabs2(tmp);
         t[1] += dabs2(tmp - lambda * rn);
       }
-      other[cl * Hk + ishift_dst(kfe, Hk)] = m * rn;
+      other[lp<NF>(cl * Hk + ishift_dst(kfe, Hk))] = m * rn;
     };
     if (pf) {
 #pragma unroll
@@ -973,7 +1007,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
     for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
       const int cl = divn<NF>(e, Hk, rHk), kfe = e - cl * Hk;
       const R m = Mc[size_t(ks0 + cl) * Hk + kfe];
-      bufA[cl * Hk + ishift_dst(kfe, Hk)] = m * in[(size_t(c) * Hk + kfe) * W + d.cols[ks0 + cl]];
+      bufA[lp<NF>(cl * Hk + ishift_dst(kfe, Hk))] = m * in[(size_t(c) * Hk + kfe) * W + d.cols[ks0 + cl]];
     }
     __syncthreads();
     res = fft_n<R, NT, true, NF>(bufA, bufB, ncg, d.planH, tw);
@@ -981,7 +1015,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   C* T2 = static_cast<C*>(d.T2) + size_t(c) * H * nc + ks0;
   for (int e = threadIdx.x; e < ncg * H; e += NT) {
     const int yy = qdiv(e, ncg, rncg), cl = e - yy * ncg;
-    T2[yy * nc + cl] = scale * res[cl * Hk + fshift_src(yy + y0, Hk)];
+    T2[yy * nc + cl] = scale * res[lp<NF>(cl * Hk + fshift_src(yy + y0, Hk))];
   }
   if (GAP) block_sum<2, NT>(t, part + size_t(li) * kGapTerms);
 }
@@ -1030,10 +1064,12 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   const int W = NF > 0 ? NF : d.W, H = d.H, nc = d.ncols;
   const int y = it.y / d.ngr_a, g = it.y - y * d.ngr_a;
   const int c0 = g * cb, ncb = min(cb, d.C - c0);
-  RowLds<C> L(smem_raw, W, cb);
+  // the wide variant keeps two blocks per CU only without the LDS padding
+  constexpr bool LP = NT != kRowAdjThreadsWide;
+  RowLds<C> L(smem_raw, W, cb, LP);
   const C* gtw = static_cast<const C*>(d.planW.tw);
   for (int e = threadIdx.x; e < W; e += NT) L.tw[e] = gtw[e];
-  for (int e = threadIdx.x; e < ncb * W; e += NT) L.a[e] = pzero<C>();
+  for (int e = threadIdx.x; e < (LP ? lds_pad(ncb * W) : ncb * W); e += NT) L.a[e] = pzero<C>();
   for (int e = threadIdx.x; e < nc; e += NT) L.cols[e] = d.cols[e];
   __syncthreads();
   const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
@@ -1058,10 +1094,10 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   const int cstride = H * nc;
   for (int e = threadIdx.x; e < ncb * nc; e += NT) {
     const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
-    L.a[cl * W + ishift_dst(L.cols[ks], W)] = T2[cl * cstride + ks];
+    L.a[lp<NF, LP>(cl * W + ishift_dst(L.cols[ks], W))] = T2[cl * cstride + ks];
   }
   __syncthreads();
-  C* res = fft_n<R, NT, true, NF>(L.a, L.b, ncb, d.planW, L.tw);
+  C* res = fft_n<R, NT, true, NF, LP>(L.a, L.b, ncb, d.planW, L.tw);
   C* other = (res == L.a) ? L.b : L.a;
   if (pf) {
 #pragma unroll
@@ -1069,22 +1105,22 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
       const int e = threadIdx.x + k * NT;
       if (e < ncb * W) {
         const int cl = divn<NF>(e, W, rW), x = e - cl * W;
-        other[cl * W + x] = mulconj(res[cl * W + fshift_src(x, W)], pS[k]);
+        other[lp<NF, LP>(cl * W + x)] = mulconj(res[lp<NF, LP>(cl * W + fshift_src(x, W))], pS[k]);
       }
     }
   } else {
     for (int e = threadIdx.x; e < ncb * W; e += NT) {
       const int cl = divn<NF>(e, W, rW), x = e - cl * W;
-      const C v = res[cl * W + fshift_src(x, W)];
-      other[cl * W + x] = Sg ? mulconj(v, Sg[cl * G + x]) : v;
+      const C v = res[lp<NF, LP>(cl * W + fshift_src(x, W))];
+      other[lp<NF, LP>(cl * W + x)] = Sg ? mulconj(v, Sg[cl * G + x]) : v;
     }
   }
   __syncthreads();
   if (d.ngr_a > 1) {
     C* wp = static_cast<C*>(d.wpart) + (size_t(g) * H + y) * W;
     for (int x = threadIdx.x; x < W; x += NT) {
-      C acc = other[x];
-      for (int cl = 1; cl < ncb; ++cl) acc = acc + other[cl * W + x];
+      C acc = other[lp<NF, LP>(x)];
+      for (int cl = 1; cl < ncb; ++cl) acc = acc + other[lp<NF, LP>(cl * W + x)];
       wp[x] = acc;
     }
     return;
@@ -1092,8 +1128,8 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   const R tau = R(d.tau), theta = R(d.theta);
   double t[2] = {0, 0};
   for (int x = threadIdx.x; x < W; x += NT) {
-    C w = other[x];
-    for (int cl = 1; cl < ncb; ++cl) w = w + other[cl * W + x];
+    C w = other[lp<NF, LP>(x)];
+    for (int cl = 1; cl < ncb; ++cl) w = w + other[lp<NF, LP>(cl * W + x)];
     if (MODE == M_OP) static_cast<C*>(d.op_out)[size_t(y) * W + x] = w;
     else mri_primal_pixel<R, MODE, GAP>(d, x, y, w, tau, theta, t);
   }
@@ -1283,13 +1319,13 @@ __global__ __launch_bounds__(kBlock) void k_zero(uint32_t* p, size_t n) {
 // ---------------------------------------------------------------------------
 #define PDT_LAUNCH(kern, grid, lds, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, __VA_ARGS__)
 
-size_t lds_rows_bytes(int dtype, int cb, int W, int nc) {
+size_t lds_rows_bytes(int dtype, int cb, int W, int nc, bool padded) {
   const size_t e = (dtype == DT_C128 || dtype == DT_F64) ? 16 : 8;
-  return (size_t(W) + 2 * size_t(cb) * W) * e + size_t(nc) * sizeof(int);
+  return (size_t(W) + 2 * size_t(padded ? lds_pad(cb * W) : cb * W)) * e + size_t(nc) * sizeof(int);
 }
 size_t lds_cols_bytes(int dtype, int cg, int Hk) {
   const size_t e = (dtype == DT_C128 || dtype == DT_F64) ? 16 : 8;
-  return (size_t(Hk) + 2 * size_t(cg) * Hk) * e;
+  return (size_t(Hk) + 2 * size_t(lds_pad(cg * Hk))) * e;
 }
 
 int num_stages(int kind) { return (kind == K_TV || kind == K_TGV) ? 2 : 4; }

@@ -66,7 +66,9 @@ void run_zero(hipStream_t st, void* ptr, size_t bytes);
 size_t init_kernel_attrs();
 
 // LDS bytes needed by the row / column kernels.
-size_t lds_rows_bytes(int dtype, int cb, int W, int nc);
+// (padded: the FFT buffers with one pad slot per 8 entries; the wide adjoint
+// row pass uses them unpadded)
+size_t lds_rows_bytes(int dtype, int cb, int W, int nc, bool padded = true);
 size_t lds_cols_bytes(int dtype, int cg, int Hk);
 
 }  // namespace pdt
