@@ -195,7 +195,7 @@ PDT_API int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, in
  * fastest in the process-wide tile cache (persisted to the file named by the
  * environment variable PDT_TUNE_CACHE, if set) and returns it.  Advances the
  * solver state: call before the solves and reset with set_input0. */
-PDT_API int pdt_autotune(pdt_solver** solvers, int count, int reps, int* cb, int* cg, int* rpi,
+PDT_API int pdt_autotune(pdt_solver** solvers, int count, int reps, int* cb, int* cg, int* rpi, int* cb_a,
                          double* ms_per_iteration);
 
 /* NEW: solve many configured solvers (same kind and dtype) together, with one

@@ -1701,7 +1701,7 @@ double time_iteration(Engine& e, int reps) {
 }
 }  // namespace
 
-int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* out_cg, int* out_rpi,
+int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* out_cg, int* out_rpi, int* out_cb_a,
                  double* out_ms) {
   return guard([&] {
     if (!solvers || count <= 0 || reps <= 0) fail(PDT_ERR_ARG, "bad argument");
@@ -1750,6 +1750,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* ou
     if (out_cb) *out_cb = bc[0];
     if (out_cg) *out_cg = bc[1];
     if (out_rpi) *out_rpi = bc[2];
+    if (out_cb_a) *out_cb_a = bc[3];
     if (out_ms) *out_ms = best;
   });
 }

@@ -109,7 +109,7 @@ def lib():
         L.pdt_solver_profile.argtypes = [vp, ct.c_int, ct.POINTER(PdtKernelProf), ct.c_int,
                                          ct.POINTER(ct.c_int)]
         ip = ct.POINTER(ct.c_int)
-        L.pdt_autotune.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ip, ip, ip, dp]
+        L.pdt_autotune.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ip, ip, ip, ip, dp]
         L.pdt_batch_solve.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int]
         fp, szp = ct.POINTER(ct.c_float), ct.POINTER(ct.c_size_t)
         L.pdt_cnn_create.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(vp)]

@@ -257,10 +257,10 @@ def autotune(optimizers, reps: int = 20) -> dict:
     for o in optimizers:
         o._push_params()
     arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
-    cb, cg, rpi, ms = ct.c_int(), ct.c_int(), ct.c_int(), ct.c_double()
+    cb, cg, rpi, cb_a, ms = ct.c_int(), ct.c_int(), ct.c_int(), ct.c_int(), ct.c_double()
     C.check(C.lib().pdt_autotune(arr, len(optimizers), int(reps), ct.byref(cb), ct.byref(cg), ct.byref(rpi),
-                                 ct.byref(ms)))
-    return {"cb": cb.value, "cg": cg.value, "rpi": rpi.value, "ms_per_iteration": ms.value}
+                                 ct.byref(cb_a), ct.byref(ms)))
+    return {"cb": cb.value, "cg": cg.value, "rpi": rpi.value, "cb_a": cb_a.value, "ms_per_iteration": ms.value}
 
 
 def batch_solve(optimizers, verbose: bool = False) -> None:
