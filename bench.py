@@ -423,7 +423,7 @@ def main():
     dom = max(prof, key=lambda p: p["ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_cfg3_tgv_mri.json")
+    tfile = os.path.join(ROOT, "profiles", "traffic_tgv_mri_320x320x15.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get(dom["name"])

@@ -210,7 +210,7 @@ def batch_cases(seeds, workers: int = 16):
         return list(ex.map(batch_case, seeds))
 
 
-def cfg3_case(seed: int):
+def mri320_case(seed: int):
     """A slice like the 320 x 320, 15-coil benchmark workload with its own seed
     (4x random lines, 24 centre lines, 30 dB)."""
     return mri_case(seed=seed, n=320, coils=15, accel=4.0, center=24, snr_db=30.0, roos=True)

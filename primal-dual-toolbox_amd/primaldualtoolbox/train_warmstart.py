@@ -70,10 +70,10 @@ def make_pairs(seeds, workers: int, chunk: int, tag: str):
     X, T, iters = [], [], []
     t0 = time.time()
     with syn.process_pool(workers) as ex:
-        pending = [ex.submit(syn.cfg3_case, s) for s in seeds[:chunk]]
+        pending = [ex.submit(syn.mri320_case, s) for s in seeds[:chunk]]
         for k in range(0, len(seeds), chunk):
             cases = [f.result() for f in pending]
-            pending = [ex.submit(syn.cfg3_case, s) for s in seeds[k + chunk:k + 2 * chunk]]
+            pending = [ex.submit(syn.mri320_case, s) for s in seeds[k + chunk:k + 2 * chunk]]
             x, t, it = solve_pairs(cases)
             del cases
             X.append(x)
@@ -142,7 +142,7 @@ def warm_vs_cold(net, seeds, workers: int, chunk: int = 64):
     cold, warm = [], []
     with syn.process_pool(workers) as ex:
         for k in range(0, len(seeds), chunk):
-            cases = list(ex.map(syn.cfg3_case, seeds[k:k + chunk]))
+            cases = list(ex.map(syn.mri320_case, seeds[k:k + chunk]))
             for use_net in (False, True):
                 built = [build_solver(c) for c in cases]
                 if use_net:

@@ -125,7 +125,7 @@ def test_trained_warm_start_reduces_iterations():
     if not os.path.exists(ws.DEFAULT_WEIGHTS):
         pytest.skip("no trained weights in the package")
     net = ws.WarmStartCNN.load()
-    case = syn.cfg3_case(3000)
+    case = syn.mri320_case(3000)
     a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
     res = {}
     for warm in (False, True):

@@ -40,7 +40,7 @@ def _mri_solver(pdt, case, cls=None, op_cls=None, dtype=np.complex64, a1=None, m
     return op, opt
 
 
-def test_cfg1_tv_256_500_iterations(oracle):
+def test_tv_256_500_iterations(oracle):
     """TV-L2 256x256 float32, lambda 10, 500 iterations, vs FP64 oracle; gap within 1 %."""
     pdt = _pdt()
     from primaldualtoolbox import synthetic as syn
@@ -53,13 +53,13 @@ def test_cfg1_tv_256_500_iterations(oracle):
     ref, _, ost, _ = oracle.solve(oracle.TV, noisy.astype(np.float64), lambda_=10, max_iter=499, gap_every=500)
     err = rel_l2(opt.getResult(), ref)
     st = opt.getStats()
-    print(f"cfg1: rel L2 {err:.3e} gap gpu {st['rel_gap']:.5e} oracle {ost['rel_gap']:.5e}", flush=True)
+    print(f"tv 256: rel L2 {err:.3e} gap gpu {st['rel_gap']:.5e} oracle {ost['rel_gap']:.5e}", flush=True)
     assert err < 1e-3
     assert abs(st["rel_gap"] - ost["rel_gap"]) <= 0.01 * ost["rel_gap"]
 
 
 @pytest.mark.parametrize("fp16", [False, True])
-def test_cfg2_tgv_complex_512_1000_iterations(oracle, fp16):
+def test_tgv_complex_512_1000_iterations(oracle, fp16):
     """TGV complex64 512x512, alpha1 1, alpha0 2, lambda 20, 1000 iterations.
     FP32 duals: rel L2 <= 1e-3; FP16-storage duals: error reported, <= 5e-2."""
     pdt = _pdt()
@@ -70,23 +70,23 @@ def test_cfg2_tgv_complex_512_1000_iterations(oracle, fp16):
     p = opt.getParameters()
     p.Lambda, p.alpha0, p.alpha1, p.max_iter, p.dual_fp16 = 20, 2, 1, 999, fp16
     opt.solve()
-    ref = _cfg2_ref(oracle, noisy)
+    ref = _tgv512_ref(oracle, noisy)
     err = rel_l2(opt.getResult(), ref)
-    print(f"cfg2 fp16={fp16}: rel L2 vs FP64 oracle {err:.3e}", flush=True)
+    print(f"tgv 512 fp16={fp16}: rel L2 vs FP64 oracle {err:.3e}", flush=True)
     assert err < (5e-2 if fp16 else 1e-3)
 
 
 _CFG2 = {}
 
 
-def _cfg2_ref(oracle, noisy):
+def _tgv512_ref(oracle, noisy):
     if "u" not in _CFG2:
         _CFG2["u"] = oracle.solve(oracle.TGV, noisy.astype(np.complex128), lambda_=20, alpha0=2, alpha1=1,
                                   max_iter=999)[0]
     return _CFG2["u"]
 
 
-def test_cfg3_full_size_parity_fixed_iterations(oracle):
+def test_mri_320_full_size_parity_fixed_iterations(oracle):
     """320x320, 15 coils, ROOS 640: 20 iterations vs the FP64 oracle."""
     pdt = _pdt()
     from primaldualtoolbox import synthetic as syn
@@ -98,7 +98,7 @@ def test_cfg3_full_size_parity_fixed_iterations(oracle):
     ref, _, _, _ = oracle.solve(oracle.TGV_MRI, case["kspace"].astype(np.complex128), op=oop, alpha1=a1,
                                 alpha0=2 * a1, max_iter=19)
     err = rel_l2(opt.getResult(), ref)
-    print(f"cfg3 20 iterations: rel L2 {err:.3e}", flush=True)
+    print(f"mri 320 20 iterations: rel L2 {err:.3e}", flush=True)
     assert err < 1e-3
 
 
@@ -166,7 +166,7 @@ def test_batch_mixed_compiled_and_runtime_lengths():
         assert np.array_equal(a.getResult(), b.getResult())
 
 
-def test_cfg3_time_to_gap_fp32_matches_fp64_gpu():
+def test_mri_320_time_to_gap_fp32_matches_fp64_gpu():
     """Size-independent check at full size: the FP32 and FP64 GPU solvers stop
     within one gap interval of each other at rel. gap 1e-4 and agree on u."""
     pdt = _pdt()
