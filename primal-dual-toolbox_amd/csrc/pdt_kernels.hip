@@ -832,7 +832,8 @@ struct RowLds {
 };
 
 // Row pass, forward.  Item = (slice, y * ngr + g): coil group g of image row
-// y.  Group 0 also performs the dual step of row y (TV or TGV).  Then for
+// y.  The groups of row y also perform the dual step of row y (TV or TGV),
+// each on its own range of pixels.  Then for
 // each coil of the group: S_c * src row -> ifftshift -> PE-FFT -> fftshift
 // -> sampled columns -> T1 [c][y][ks].
 template <class R, int MODE, bool GAP, int NT, int NF>
@@ -880,13 +881,17 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
     }
   }
   double t[kGapTerms] = {0};
-  if (MODE != M_OP && g == 0) {
+  if (MODE != M_OP) {
+    // The dual step of row y is shared out over the row's coil groups in
+    // pixel ranges of a multiple of 32, so no group carries all of it.
+    const int chunk = ((W + d.ngr - 1) / d.ngr + 31) & ~31;
+    const int xa = min(W, g * chunk), xe = min(W, xa + chunk);
     const int k = d.ctl[CTL_ITER];
     const R sigma = R(d.sigma);
     const R a0 = sched_alpha<R>(d, d.alpha0, d.alpha0_end, k);
     const R a1 = sched_alpha<R>(d, d.alpha1, d.alpha1_end, k);
     const R ia0 = rcp(a0), ia1 = rcp(a1);
-    for (int x = threadIdx.x; x < W; x += NT) dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, ia0, ia1, t);
+    for (int x = xa + threadIdx.x; x < xe; x += NT) dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, ia0, ia1, t);
   }
   if (pf) {
 #pragma unroll
@@ -1090,6 +1095,28 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
       }
     }
   }
+  // With all coils of the row in one block of at least 2 W threads, the
+  // parts of the primal step that do not need A^* r run before the FFT, so
+  // their loads overlap those of S and T2: threads x < W load u(x, y) and
+  // form div p (p is final for this iteration), threads W + x perform the
+  // whole v step of pixel x (TGV_primal_v_kernel, tgv_kernels.cuh:647-686,
+  // needs only p and q).
+  const R tau = R(d.tau), theta = R(d.theta);
+  double t[2] = {0, 0};
+  const bool early = NF > 0 && MODE != M_OP && d.ngr_a == 1 && 2 * W <= NT;
+  C e_uo = pzero<C>(), e_div = pzero<C>();
+  if (early) {
+    const int x = threadIdx.x;
+    if (x < W) {
+      const size_t n = size_t(W) * H, i = size_t(y) * W + x;
+      const C* px = static_cast<const C*>(d.p);
+      e_uo = static_cast<const C*>(d.u)[i];
+      e_div = dmx<C>(px, x, i, W) + dmy<C>(px + n, y, i, W, H);
+    } else if (MODE == M_TGV && x < 2 * W) {
+      const double rv = vstep_pixel<C, C>(d, x - W, y, tau, theta);
+      if (GAP) t[1] += rv;
+    }
+  }
   const C* T2 = static_cast<const C*>(d.T2) + (size_t(c0) * H + y) * nc;  // row y of coil c0
   const int cstride = H * nc;
   for (int e = threadIdx.x; e < ncb * nc; e += NT) {
@@ -1125,13 +1152,24 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
     }
     return;
   }
-  const R tau = R(d.tau), theta = R(d.theta);
-  double t[2] = {0, 0};
-  for (int x = threadIdx.x; x < W; x += NT) {
-    C w = other[lp<NF, LP>(x)];
-    for (int cl = 1; cl < ncb; ++cl) w = w + other[lp<NF, LP>(cl * W + x)];
-    if (MODE == M_OP) static_cast<C*>(d.op_out)[size_t(y) * W + x] = w;
-    else mri_primal_pixel<R, MODE, GAP>(d, x, y, w, tau, theta, t);
+  if (early) {
+    const int x = threadIdx.x;
+    if (x < W) {
+      C w = other[lp<NF, LP>(x)];
+      for (int cl = 1; cl < ncb; ++cl) w = w + other[lp<NF, LP>(cl * W + x)];
+      const size_t i = size_t(y) * W + x;
+      const C un = e_uo + tau * (e_div - w);
+      static_cast<C*>(d.u)[i] = un;
+      static_cast<C*>(d.ub)[i] = (R(1) + theta) * un - theta * e_uo;
+      if (GAP) t[0] += dabs2(w - e_div);
+    }
+  } else {
+    for (int x = threadIdx.x; x < W; x += NT) {
+      C w = other[lp<NF, LP>(x)];
+      for (int cl = 1; cl < ncb; ++cl) w = w + other[lp<NF, LP>(cl * W + x)];
+      if (MODE == M_OP) static_cast<C*>(d.op_out)[size_t(y) * W + x] = w;
+      else mri_primal_pixel<R, MODE, GAP>(d, x, y, w, tau, theta, t);
+    }
   }
   if (MODE == M_OP) return;
   if (GAP) block_sum<2, NT>(t, part_rows + size_t(d.row_beg + y) * kGapTerms);
