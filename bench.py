@@ -234,6 +234,49 @@ def extra_tgv512(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
     return out
 
 
+def extra_demo384(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
+    """The geometry of the reference's Cartesian demo (bin/mri_cartesian_test.py):
+    384x384 image on the 768 FE x 384 PE oversampled grid, the reference's
+    random4 mask (117 PE lines, as stored in tests/golden/masks.npz), 15
+    synthetic coils, alpha as in the headline workload.  Time-to-gap 1e-4 from
+    the cold start, and the demo's fixed 1001 iterations (max_iter 1000)."""
+    from primaldualtoolbox import _capi as C
+    from primaldualtoolbox import synthetic as syn
+    lines = np.load(os.path.join(ROOT, "tests", "golden", "masks.npz"))["random4_lines"]
+    case = syn.mri_case(seed=0, n=384, coils=15, n_pe=384, lines=lines)
+    a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
+    import primaldualtoolbox as pdt
+    op, opt = build_solver(case, a1)
+    u0 = C.DeviceArray.from_numpy(op.adjoint(case["kspace"]), np.complex64)
+    tiles = pdt.mri.autotune([opt], reps=10)
+    opt.setNoisyData(case["kspace"])
+    opt.solve()  # first solve: engine, graphs
+    s = _timed_solves(opt, u0, steps)
+    st = opt.getStats()
+    if not st["converged"]:
+        raise RuntimeError(f"demo geometry did not reach rel. gap {GAP_TOL}: {st}")
+    it = int(st["iters"])
+    p = opt.getParameters()
+    p.max_iter, p.gap_tol, p.gap_every = 1000, 0.0, 0
+    s_fixed = _timed_solves(opt, u0, steps)
+    out = {"workload": ("TGV-MRI 384x384 complex64, 15 coils, ROOS grid 384 PE x 768 FE, reference random4 "
+                        "mask (117 PE lines), 30 dB SNR, alpha1=1e-5||f||, alpha0=2 alpha1, cold start"),
+           "seconds_to_gap": s, "iterations_to_gap": it, "us_per_iteration": s * 1e6 / it,
+           "seconds_1001_iterations": s_fixed, "tiles": tiles,
+           "kernels": {k["name"]: round(k["ms"], 5) for k in opt.profileKernels(50)}}
+    if not no_cpu:
+        O = _oracle()
+        O.set_threads(cpu_threads)
+        n_it = 20
+        oop = O.Op(O.OP_CARTESIAN_ROOS, case["sens"], case["mask"], prec=32)
+        _, _, ost, _ = O.solve(O.TGV_MRI, case["kspace"], op=oop, alpha1=a1, alpha0=2 * a1, max_iter=n_it - 1,
+                               prec=32)
+        out["cpu_baseline"] = {"value": ost["seconds"] * it / n_it, "unit": "s", "cores": cpu_threads,
+                               "kind": "port",
+                               "sample": f"{n_it} iterations in the FP32 OpenMP oracle, scaled to {it}"}
+    return out
+
+
 def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool) -> dict:
     """`count` mixed Cartesian TGV-MRI slices (sizes 256^2..384x320, 8-20
     coils, random or regular masks) solved together to rel. gap 1e-4 in one
@@ -442,6 +485,7 @@ def main():
         for key, fn in (("cnn", extra_cnn),
                         ("tv_256", lambda: extra_tv256(3, args.cpu_threads, args.no_cpu)),
                         ("tgv_512", lambda: extra_tgv512(3, args.cpu_threads, args.no_cpu)),
+                        ("demo_384", lambda: extra_demo384(3, args.cpu_threads, args.no_cpu)),
                         (f"batch_{args.batch}", lambda: extra_batch64(args.batch, args.gen_workers,
                                                                       args.cpu_threads, args.no_cpu))):
             log(f"[bench] side measurement {key}")

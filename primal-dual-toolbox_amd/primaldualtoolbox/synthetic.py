@@ -147,20 +147,25 @@ def cartesian_forward_np(x: np.ndarray, sens: np.ndarray, mask: np.ndarray,
 
 def mri_case(seed: int = 0, n: int = 320, coils: int = 15, n_pe: int | None = None,
              accel: float = 4.0, center: int = 24, regular: bool = False,
-             snr_db: float = 30.0, roos: bool = True):
+             snr_db: float = 30.0, roos: bool = True, lines=None):
     """Cartesian multi-coil case on a read-out oversampled grid.
 
     Returns a dict with ``image`` (n, n_pe) complex, ``sens`` (coils, 2n, n_pe)
     complex64, ``mask`` (2n, n_pe) float32 and ``kspace`` (coils, 2n, n_pe)
     complex64 = mask * (A x + noise) at the requested SNR
-    (20 log10 ||A x|| / ||noise|| over sampled entries).
+    (20 log10 ||A x|| / ||noise|| over sampled entries).  ``lines`` (PE-line
+    indices) replaces the random mask, e.g. by one of the reference's masks.
     """
     n_pe = n if n_pe is None else n_pe
     rng = np.random.default_rng(seed)
     img = complex_phantom(rng, n_pe, n)
     n_fe_k = 2 * n if roos else n
     sens = coil_sensitivities(rng, coils, n_fe_k, n_pe, size=min(n, n_pe))
-    mask = line_mask(rng, n_fe_k, n_pe, accel=accel, center=center, regular=regular)
+    if lines is None:
+        mask = line_mask(rng, n_fe_k, n_pe, accel=accel, center=center, regular=regular)
+    else:
+        mask = np.zeros((n_fe_k, n_pe), np.float32)
+        mask[:, np.asarray(lines, dtype=np.int64)] = 1.0
     y = cartesian_forward_np(img, sens, mask, roos)
     sampled = np.broadcast_to(mask[None] != 0, y.shape)
     noise = rng.standard_normal(y.shape) + 1j * rng.standard_normal(y.shape)
