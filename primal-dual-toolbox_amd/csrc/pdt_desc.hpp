@@ -16,7 +16,7 @@ constexpr int kRowThreads = 256;  // threads per workgroup of the MRI row FFT pa
 constexpr int kRowAdjThreadsWide = 1024;  // adjoint row pass with many coils per item
 constexpr int kColThreads = 256;  // threads per workgroup of the MRI column pass
 constexpr int kPrefetch = 8;       // row-pass elements per thread held in registers ahead of use
-constexpr int kColPrefetchCols = 4;  // column groups up to this width prefetch their prox operands
+constexpr int kColPrefetchCols = 2;  // column groups up to this width prefetch their prox operands
 constexpr int kMaxStages = 14;    // FFT stages per transform
 constexpr int kGapTerms = 8;      // partial-sum slots per work item
 constexpr int kMaxRowLen = 4096;  // W limit of the MRI row kernels

@@ -270,20 +270,63 @@ __device__ void butterfly_gen(const Cx<R>* in, Cx<R>* out, int r, int nr, int p,
   }
 }
 
+// cos and sin of 2 pi j / n at compile time (Taylor series on |x| <= pi / 2
+// after the symmetries of the circle), for the odd-prime butterflies.
+constexpr double ct_taylor_sin(double x) {
+  double t = x, r = x;
+  for (int k = 1; k < 14; ++k) {
+    t *= -x * x / double((2 * k) * (2 * k + 1));
+    r += t;
+  }
+  return r;
+}
+constexpr double ct_taylor_cos(double x) {
+  double t = 1, r = 1;
+  for (int k = 1; k < 14; ++k) {
+    t *= -x * x / double((2 * k - 1) * (2 * k));
+    r += t;
+  }
+  return r;
+}
+constexpr double ct_sincos2pi(int j, int n, bool want_sin) {
+  j %= n;
+  double sg = 1;
+  if (2 * j > n) {  // angle in (pi, 2 pi): sin changes sign, cos does not
+    j = n - j;
+    if (want_sin) sg = -1;
+  }
+  constexpr double kPi = 3.14159265358979323846264338327950288;
+  double x = 2 * kPi * double(j) / double(n);  // [0, pi]
+  if (2 * x > kPi) {                          // (pi / 2, pi]: reflect about pi / 2
+    x = kPi - x;
+    return want_sin ? sg * ct_taylor_sin(x) : -ct_taylor_cos(x);
+  }
+  return want_sin ? sg * ct_taylor_sin(x) : ct_taylor_cos(x);
+}
+template <int P>
+struct OddRoots {
+  double c[P] = {}, s[P] = {};
+  constexpr OddRoots() {
+    for (int j = 0; j < P; ++j) {
+      c[j] = ct_sincos2pi(j, P, false);
+      s[j] = ct_sincos2pi(j, P, true);
+    }
+  }
+};
+template <int P>
+constexpr OddRoots<P> kOddRoots{};
+
 // Odd radix P (7, 11, 13, 23, ...) in the symmetric form: with
 // s_m = x_m + x_(P-m), d_m = x_m - x_(P-m), m = 1..h, h = (P-1)/2,
 //   X_0 = x_0 + sum_m s_m,
 //   X_k, X_(P-k) = x_0 + sum_m cos(2 pi m k / P) s_m  +/-  i sg sum_m sin(2 pi m k / P) d_m,
-// sg = -1 forward, +1 inverse.  The coefficients come from the length-n
-// twiddle table, tw[(m k mod P) nr] = (cos, -sin)(2 pi m k / P), and the
-// loop over output pairs stays rolled: only x_0, s and d are live across it,
-// which keeps this rarely used radix from raising the register count of the
-// kernels that contain it.  h^2 complex-by-real products per output pair.
-// in[j * nr] are the inputs, out[q * p] the outputs; tw[m * tstep] =
-// e^(-2 pi i m / P) (tstep = n / P of the length-n table).
+// sg = -1 forward, +1 inverse: h^2 complex-by-real products per output pair.
+// Both loops unroll, so the coefficients are compile-time constants rather
+// than loads from the twiddle table.  in[j * nr] are the inputs, out[q * p]
+// the outputs; tw (length-n table) supplies the stage twiddles only.
 template <class R, int P>
 __device__ __forceinline__ void butterfly_odd(const Cx<R>* in, Cx<R>* out, int nr, int p, const Cx<R>* tw,
-                                              int ts, bool inv, int tstep) {
+                                              int ts, bool inv) {
   constexpr int h = (P - 1) / 2;
   const Cx<R> x0 = in[0];
   Cx<R> s[h], d[h];
@@ -298,17 +341,14 @@ __device__ __forceinline__ void butterfly_odd(const Cx<R>* in, Cx<R>* out, int n
   }
   out[0] = sum;
   const R sg = inv ? R(1) : R(-1);
-#pragma unroll 1
+#pragma unroll
   for (int k = 1; k <= h; ++k) {
     Cx<R> A = x0, B = pzero<Cx<R>>();
-    int jk = 0;
 #pragma unroll
     for (int m = 1; m <= h; ++m) {
-      jk += k;
-      if (jk >= P) jk -= P;
-      const Cx<R> w = tw[jk * tstep];
-      A = A + s[m - 1] * w.re;
-      B = B - d[m - 1] * w.im;
+      const int jk = (m * k) % P;
+      A = A + s[m - 1] * R(kOddRoots<P>.c[jk]);
+      B = B + d[m - 1] * R(kOddRoots<P>.s[jk]);
     }
     const Cx<R> iB = muli(B, sg);
     out[k * p] = A + iB;
@@ -423,10 +463,10 @@ __device__ __forceinline__ int lp(int e) {
 }
 
 // TW: stage twiddles present (false for the first stage, where they are 1).
-// nr, p: LDS strides of the inputs and outputs; tstep = n / RR.
+// nr, p: LDS strides of the inputs and outputs.
 template <class R, int RR, bool TW>
 __device__ __forceinline__ void butterfly_ct(const Cx<R>* in, Cx<R>* out, int ts, const Cx<R>* tw, bool inv,
-                                             int nr, int p, int tstep) {
+                                             int nr, int p) {
   if constexpr (RR == 2 || RR == 3 || RR == 4 || RR == 5 || RR == 8) {
     Cx<R> x[RR];
     x[0] = in[0];
@@ -437,7 +477,7 @@ __device__ __forceinline__ void butterfly_ct(const Cx<R>* in, Cx<R>* out, int ts
     for (int q = 0; q < RR; ++q) out[q * p] = x[q];
   } else {
     static_assert(RR % 2 == 1, "even radices are factored into 2, 4 and 8");
-    butterfly_odd<R, RR>(in, out, nr, p, tw, ts, inv, tstep);
+    butterfly_odd<R, RR>(in, out, nr, p, tw, ts, inv);
   }
 }
 
@@ -458,7 +498,7 @@ __device__ __forceinline__ Cx<R>* fft_ct_stage(Cx<R>* a, Cx<R>* b, int nseq, con
       const int k = i % P;
       const int o = (i - k) * r + k;
       const int ip = PAD ? i + (i >> 3) : i, op = PAD ? o + (o >> 3) : o;
-      butterfly_ct<R, r, (P > 1)>(a + seq * NP + ip, b + seq * NP + op, k * tws, tw, INV, nrp, pp, nr);
+      butterfly_ct<R, r, (P > 1)>(a + seq * NP + ip, b + seq * NP + op, k * tws, tw, INV, nrp, pp);
     }
     __syncthreads();
     return fft_ct_stage<R, N, S + 1, P * r, NT, INV, LP>(b, a, nseq, tw);
