@@ -323,10 +323,12 @@ constexpr OddRoots<P> kOddRoots{};
 // sg = -1 forward, +1 inverse: h^2 complex-by-real products per output pair.
 // Both loops unroll, so the coefficients are compile-time constants rather
 // than loads from the twiddle table.  in[j * nr] are the inputs, out[q * p]
-// the outputs; tw (length-n table) supplies the stage twiddles only.
+// the outputs; tw (length-n table) supplies the stage twiddles only.  With
+// G > 1 the call computes only the output pairs k = g + 1 (mod G) (and X_0
+// for g = 0): G callers share one butterfly.
 template <class R, int P>
 __device__ __forceinline__ void butterfly_odd(const Cx<R>* in, Cx<R>* out, int nr, int p, const Cx<R>* tw,
-                                              int ts, bool inv) {
+                                              int ts, bool inv, int G = 1, int g = 0) {
   constexpr int h = (P - 1) / 2;
   const Cx<R> x0 = in[0];
   Cx<R> s[h], d[h];
@@ -339,10 +341,11 @@ __device__ __forceinline__ void butterfly_odd(const Cx<R>* in, Cx<R>* out, int n
     d[m - 1] = a - b;
     sum = sum + s[m - 1];
   }
-  out[0] = sum;
+  if (g == 0) out[0] = sum;
   const R sg = inv ? R(1) : R(-1);
 #pragma unroll
   for (int k = 1; k <= h; ++k) {
+    if (((k - 1) & (G - 1)) != g) continue;  // G is 1, 2 or 4
     Cx<R> A = x0, B = pzero<Cx<R>>();
 #pragma unroll
     for (int m = 1; m <= h; ++m) {
@@ -492,13 +495,38 @@ __device__ __forceinline__ Cx<R>* fft_ct_stage(Cx<R>* a, Cx<R>* b, int nseq, con
     constexpr bool PAD = LP && ct_pad_ok(N);
     constexpr int NP = PAD ? N + N / 8 : N, nrp = PAD ? nr + nr / 8 : nr, pp = PAD && P > 1 ? P + P / 8 : P;
     const int total = nseq * nr;
-    for (int t = threadIdx.x; t < total; t += NT) {
-      const int seq = t / nr;
-      const int i = t - seq * nr;
-      const int k = i % P;
-      const int o = (i - k) * r + k;
-      const int ip = PAD ? i + (i >> 3) : i, op = PAD ? o + (o >> 3) : o;
-      butterfly_ct<R, r, (P > 1)>(a + seq * NP + ip, b + seq * NP + op, k * tws, tw, INV, nrp, pp);
+    if constexpr (r > 8) {
+      // Odd prime radix: a long chain of products per butterfly and few
+      // butterflies per block, so the output pairs of each butterfly are
+      // split over G waves (G = 4, 2 or 1, as many as keep every butterfly
+      // in one pass).  Lane l of a wave takes butterfly 64 c + l and the
+      // wave-uniform part g, so every part keeps constant coefficients.
+      constexpr int NW = NT / 64;
+      const int chunks = (total + 63) / 64;
+      const int G = chunks * 4 <= NW ? 4 : chunks * 2 <= NW ? 2 : 1;
+      const int lane = threadIdx.x & 63;
+      const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // a scalar, so the parts branch per wave
+      for (int q = wave; q < chunks * G; q += NW) {
+        const int c = q / G, g = q - c * G;
+        const int t = c * 64 + lane;
+        if (t < total) {
+          const int seq = t / nr;
+          const int i = t - seq * nr;
+          const int k = i % P;
+          const int o = (i - k) * r + k;
+          const int ip = PAD ? i + (i >> 3) : i, op = PAD ? o + (o >> 3) : o;
+          butterfly_odd<R, r>(a + seq * NP + ip, b + seq * NP + op, nrp, pp, tw, k * tws, INV, G, g);
+        }
+      }
+    } else {
+      for (int t = threadIdx.x; t < total; t += NT) {
+        const int seq = t / nr;
+        const int i = t - seq * nr;
+        const int k = i % P;
+        const int o = (i - k) * r + k;
+        const int ip = PAD ? i + (i >> 3) : i, op = PAD ? o + (o >> 3) : o;
+        butterfly_ct<R, r, (P > 1)>(a + seq * NP + ip, b + seq * NP + op, k * tws, tw, INV, nrp, pp);
+      }
     }
     __syncthreads();
     return fft_ct_stage<R, N, S + 1, P * r, NT, INV, LP>(b, a, nseq, tw);
