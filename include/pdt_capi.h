@@ -198,6 +198,12 @@ PDT_API int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, in
 PDT_API int pdt_autotune(pdt_solver** solvers, int count, int reps, int* cb, int* cg, int* rpi, int* cb_a,
                          double* ms_per_iteration);
 
+/* NEW: a manual tile choice for the shape of these solvers (one batch),
+ * stored in the same tile cache as pdt_autotune's result; values <= 0 take
+ * the default, values above a shape's limit are clamped.  The next solve of
+ * the same shape uses them. */
+PDT_API int pdt_set_tiles(pdt_solver** solvers, int count, int cb, int cg, int rpi, int cb_a);
+
 /* NEW: solve many configured solvers (same kind and dtype) together, with one
  * persistent work-list launch per half-iteration for the whole batch.  Each
  * solver keeps its own parameters, sizes, coils and stopping. */

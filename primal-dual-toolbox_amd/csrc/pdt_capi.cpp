@@ -1755,6 +1755,25 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* ou
   });
 }
 
+int pdt_set_tiles(pdt_solver** solvers, int count, int cb, int cg, int rpi, int cb_a) {
+  return guard([&] {
+    if (!solvers || count <= 0) fail(PDT_ERR_ARG, "bad argument");
+    std::vector<pdt_solver*> sv(solvers, solvers + count);
+    for (auto* s : sv) prepare_solve(s, 0);
+    Engine e;
+    e.sv = sv;
+    e.plan_only = true;
+    // zero or negative: the default, not a cached choice
+    e.force_cb = cb > 0 ? cb : kDefaultCb;
+    e.force_cg = cg > 0 ? cg : kDefaultCg;
+    e.force_rpi = rpi > 0 ? rpi : 1;
+    e.force_cb_a = cb_a > 0 ? cb_a : e.force_cb;
+    e.build();
+    const bool mri = sv[0]->is_op();
+    tune_store(e.key, mri ? TuneVal{e.L.cb, e.L.cg, 0, e.L.cb_a} : TuneVal{0, 0, e.L.rpi, 0});
+  });
+}
+
 int pdt_batch_solve(pdt_solver** solvers, int count, int verbose) {
   return guard([&] {
     if (!solvers || count <= 0) fail(PDT_ERR_ARG, "empty batch");

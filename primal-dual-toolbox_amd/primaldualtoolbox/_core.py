@@ -263,6 +263,17 @@ def autotune(optimizers, reps: int = 20) -> dict:
     return {"cb": cb.value, "cg": cg.value, "rpi": rpi.value, "cb_a": cb_a.value, "ms_per_iteration": ms.value}
 
 
+def set_tiles(optimizers, cb: int = 0, cg: int = 0, rpi: int = 0, cb_a: int = 0) -> None:
+    """Stores a manual tile choice for the shape of these optimizers (one
+    batch) in the tile cache the autotuner uses: coils per forward and
+    adjoint row item (cb, cb_a), columns per column item (cg), rows per
+    denoising item (rpi); 0 takes the default."""
+    for o in optimizers:
+        o._push_params()
+    arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
+    C.check(C.lib().pdt_set_tiles(arr, len(optimizers), int(cb), int(cg), int(rpi), int(cb_a)))
+
+
 def batch_solve(optimizers, verbose: bool = False) -> None:
     """Solves configured optimizers of one class together, one persistent
     work-list launch per half-iteration for the whole batch."""

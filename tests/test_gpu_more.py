@@ -122,6 +122,31 @@ def test_fft_lengths_solver_parity(oracle, n, n_pe):
     assert err < 1e-3
 
 
+@pytest.mark.parametrize("cb,cg,cb_a", [(1, 1, 1), (5, 4, 10), (10, 8, 10), (10, 2, 5)])
+def test_odd_radix_split_over_waves(oracle, cb, cg, cb_a):
+    """The radix-23 stage of the 368 / 736 transforms splits each butterfly's
+    output pairs over 4, 2 or 1 waves depending on how many butterflies a
+    block holds; these tiles reach every split in the forward rows (16 cb
+    butterflies on 256 threads), the columns (32 cg) and the adjoint rows
+    (1024 threads when cb_a = 10 coils).  10 iterations vs the FP64 oracle."""
+    pdt = _pdt()
+    from primaldualtoolbox import synthetic as syn
+    case = syn.mri_case(seed=6, n=368, coils=10)
+    a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
+    _, opt = _mri_solver(pdt, case, a1=a1, max_iter=9)
+    pdt.mri.set_tiles([opt], cb=cb, cg=cg, cb_a=cb_a)
+    try:
+        opt.solve()
+    finally:
+        pdt.mri.set_tiles([opt])  # back to the default tiles for this shape
+    oop = oracle.Op(oracle.OP_CARTESIAN_ROOS, case["sens"].astype(np.complex128), case["mask"].astype(np.float64))
+    ref, _, _, _ = oracle.solve(oracle.TGV_MRI, case["kspace"].astype(np.complex128), op=oop, alpha1=a1,
+                                alpha0=2 * a1, max_iter=9)
+    err = rel_l2(opt.getResult(), ref)
+    print(f"tiles cb={cb} cg={cg} cb_a={cb_a}: rel L2 {err:.3e}", flush=True)
+    assert err < 1e-3
+
+
 def _batch_pair(pdt, specs):
     """Two identical lists of configured solvers (for single vs batched)."""
     from primaldualtoolbox import synthetic as syn
