@@ -87,20 +87,42 @@ template <> struct Half<Cx<float>> { using T = Cx<_Float16>; };
 template <> struct Half<double> { using T = double; };
 template <> struct Half<Cx<double>> { using T = Cx<double>; };
 
+// Device memory reached through the slice descriptors is addressed with
+// global-address-space pointers (gptr): loads and stores through them compile
+// to global_* instructions with a scalar base, where generic pointers give
+// flat_* ones, which also count against the LDS wait counter (so every LDS
+// wait of an FFT stage would wait for the outstanding register prefetches).
+// (The host pass of the same source, which only checks device code, keeps
+// generic pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T> using G = __attribute__((address_space(1))) T;
+template <class T> struct NoAS { using type = T; };
+template <class T> struct NoAS<G<T>> { using type = T; };
+#else
+template <class T> using G = T;
+template <class T> struct NoAS { using type = T; };
+#endif
+template <class T> using Plain = std::remove_const_t<typename NoAS<T>::type>;
+template <class T>
+__device__ __forceinline__ G<T>* gptr(const void* p) {
+  return (G<T>*)(const_cast<void*>(p));
+}
+
+// Element access of a plane stored as S (get/put take any pointer to S).
 template <class P, class S> struct Ld {
-  static __device__ __forceinline__ P get(const S* a, size_t i) { return a[i]; }
-  static __device__ __forceinline__ void put(S* a, size_t i, P v) { a[i] = v; }
+  template <class A> static __device__ __forceinline__ P get(A* a, size_t i) { return a[i]; }
+  template <class A> static __device__ __forceinline__ void put(A* a, size_t i, P v) { a[i] = v; }
 };
 template <> struct Ld<float, _Float16> {
-  static __device__ __forceinline__ float get(const _Float16* a, size_t i) { return float(a[i]); }
-  static __device__ __forceinline__ void put(_Float16* a, size_t i, float v) { a[i] = _Float16(v); }
+  template <class A> static __device__ __forceinline__ float get(A* a, size_t i) { return float(a[i]); }
+  template <class A> static __device__ __forceinline__ void put(A* a, size_t i, float v) { a[i] = _Float16(v); }
 };
 template <> struct Ld<Cx<float>, Cx<_Float16>> {
-  static __device__ __forceinline__ Cx<float> get(const Cx<_Float16>* a, size_t i) {
+  template <class A> static __device__ __forceinline__ Cx<float> get(A* a, size_t i) {
     Cx<_Float16> h = a[i];
     return {float(h.re), float(h.im)};
   }
-  static __device__ __forceinline__ void put(Cx<_Float16>* a, size_t i, Cx<float> v) {
+  template <class A> static __device__ __forceinline__ void put(A* a, size_t i, Cx<float> v) {
     a[i] = Cx<_Float16>{_Float16(v.re), _Float16(v.im)};
   }
 };
@@ -566,12 +588,12 @@ __device__ __forceinline__ int ishift_src(int j, int n) {
 // ---------------------------------------------------------------------------
 template <class P, class S>
 __device__ __forceinline__ P dmx(const S* a, int x, size_t i, int W) {
-  using L = Ld<P, S>;
+  using L = Ld<P, Plain<S>>;
   return (x > 0) ? ((x < W - 1) ? L::get(a, i) - L::get(a, i - 1) : -L::get(a, i - 1)) : L::get(a, i);
 }
 template <class P, class S>
 __device__ __forceinline__ P dmy(const S* a, int y, size_t i, int W, int H) {
-  using L = Ld<P, S>;
+  using L = Ld<P, Plain<S>>;
   return (y > 0) ? ((y < H - 1) ? L::get(a, i) - L::get(a, i - W) : -L::get(a, i - W)) : L::get(a, i);
 }
 
@@ -602,23 +624,23 @@ __device__ __forceinline__ void dual_pixel(const SliceDev& d, int x, int y, type
   const size_t n = size_t(W) * H;
   const size_t i = size_t(y) * W + x;
   const size_t ixp = i + (x < W - 1), iyp = i + ((y < H - 1) ? W : 0);
-  const P* ub = static_cast<const P*>(d.ub);
-  S* px = static_cast<S*>(d.p);
-  S* py = px + n;
+  const auto ub = gptr<const P>(d.ub);
+  const auto px = gptr<S>(d.p);
+  const auto py = px + n;
   const P c = ub[i];
   P tx = ub[ixp] - c, ty = ub[iyp] - c;
   if (TGV) {
-    const P* vbx = static_cast<const P*>(d.vb);
-    const P* vby = vbx + n;
+    const auto vbx = gptr<const P>(d.vb);
+    const auto vby = vbx + n;
     const P vx0 = vbx[i], vy0 = vby[i];
     tx = tx - vx0;
     ty = ty - vy0;
     const P exx = vbx[ixp] - vx0;
     const P eyy = vby[iyp] - vy0;
     const P exy = R(0.5) * (vbx[iyp] - vx0 + vby[ixp] - vy0);
-    S* qxx = static_cast<S*>(d.q);
-    S* qyy = qxx + n;
-    S* qxy = qyy + n;
+    const auto qxx = gptr<S>(d.q);
+    const auto qyy = qxx + n;
+    const auto qxy = qyy + n;
     const P bxx = L::get(qxx, i) + sigma * exx;
     const P byy = L::get(qyy, i) + sigma * eyy;
     const P bxy = L::get(qxy, i) + sigma * exy;
@@ -656,15 +678,15 @@ __device__ __forceinline__ double vstep_pixel(const SliceDev& d, int x, int y, t
   const int W = d.W, H = d.H;
   const size_t n = size_t(W) * H;
   const size_t i = size_t(y) * W + x;
-  const S* px = static_cast<const S*>(d.p);
-  const S* py = px + n;
-  const S* qxx = static_cast<const S*>(d.q);
-  const S* qyy = qxx + n;
-  const S* qxy = qyy + n;
-  P* vx = static_cast<P*>(d.v);
-  P* vy = vx + n;
-  P* vbx = static_cast<P*>(d.vb);
-  P* vby = vbx + n;
+  const auto px = gptr<const S>(d.p);
+  const auto py = px + n;
+  const auto qxx = gptr<const S>(d.q);
+  const auto qyy = qxx + n;
+  const auto qxy = qyy + n;
+  const auto vx = gptr<P>(d.v);
+  const auto vy = vx + n;
+  const auto vbx = gptr<P>(d.vb);
+  const auto vby = vbx + n;
   const P dsx = dmx<P>(qxx, x, i, W) + dmy<P>(qxy, y, i, W, H);
   const P dsy = dmy<P>(qyy, y, i, W, H) + dmx<P>(qxy, x, i, W);
   const P pxi = L::get(px, i), pyi = L::get(py, i);
@@ -678,7 +700,7 @@ __device__ __forceinline__ double vstep_pixel(const SliceDev& d, int x, int y, t
   return dabs2(pxi + dsx) + dabs2(pyi + dsy);
 }
 
-__device__ __forceinline__ bool slice_active(const SliceDev& d) { return d.ctl[CTL_ACTIVE] != 0; }
+__device__ __forceinline__ bool slice_active(const SliceDev& d) { return gptr<const int>(d.ctl)[CTL_ACTIVE] != 0; }
 
 // ---------------------------------------------------------------------------
 // Denoising half-iterations.  Item = (slice, first row); rpi rows per item.
@@ -695,8 +717,8 @@ __global__ __launch_bounds__(kBlock) void k_tv_dual(const SliceDev* sl, const in
   const int W = d.W, H = d.H;
   const int ya = it.y, yb = min(H, ya + rpi);
   const R sigma = R(d.sigma);
-  const P* ub = static_cast<const P*>(d.ub);
-  const P* f = static_cast<const P*>(d.f);
+  const auto ub = gptr<const P>(d.ub);
+  const auto f = gptr<const P>(d.f);
   double t[kGapTerms] = {0};
   const int cnt = (yb - ya) * W;
   for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
@@ -724,11 +746,11 @@ __global__ __launch_bounds__(kBlock) void k_tv_primal(const SliceDev* sl, const 
   const size_t n = size_t(W) * H;
   const int ya = it.y, yb = min(H, ya + rpi);
   const R tau = R(d.tau), lambda = R(d.lambda), theta = R(d.theta);
-  P* u = static_cast<P*>(d.u);
-  P* ub = static_cast<P*>(d.ub);
-  const P* f = static_cast<const P*>(d.f);
-  const P* px = static_cast<const P*>(d.p);
-  const P* py = px + n;
+  const auto u = gptr<P>(d.u);
+  const auto ub = gptr<P>(d.ub);
+  const auto f = gptr<const P>(d.f);
+  const auto px = gptr<const P>(d.p);
+  const auto py = px + n;
   double t[2] = {0, 0};
   const int cnt = (yb - ya) * W;
   for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
@@ -747,7 +769,7 @@ __global__ __launch_bounds__(kBlock) void k_tv_primal(const SliceDev* sl, const 
     }
   }
   if (GAP) block_sum<2>(t, part + size_t(blockIdx.x) * kGapTerms);
-  if (ya == 0 && threadIdx.x == 0) d.ctl[CTL_ITER] += 1;
+  if (ya == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += 1;
 }
 
 // TGV denoising primal half (tgvoptimizer.cu:119-127): TV_primal_u_kernel
@@ -762,11 +784,11 @@ __global__ __launch_bounds__(kBlock) void k_tgv_primal(const SliceDev* sl, const
   const size_t n = size_t(W) * H;
   const int ya = it.y, yb = min(H, ya + rpi);
   const R tau = R(d.tau), lambda = R(d.lambda), theta = R(d.theta);
-  P* u = static_cast<P*>(d.u);
-  P* ub = static_cast<P*>(d.ub);
-  const P* f = static_cast<const P*>(d.f);
-  const S* px = static_cast<const S*>(d.p);
-  const S* py = px + n;
+  const auto u = gptr<P>(d.u);
+  const auto ub = gptr<P>(d.ub);
+  const auto f = gptr<const P>(d.f);
+  const auto px = gptr<const S>(d.p);
+  const auto py = px + n;
   const int cnt = (yb - ya) * W;
   for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
     const int yy = idx / W;
@@ -793,8 +815,8 @@ __global__ __launch_bounds__(kBlock) void k_tgv_dual(const SliceDev* sl, const i
   const int W = d.W, H = d.H;
   const int ya = it.y, yb = min(H, ya + rpi);
   const R sigma = R(d.sigma), a0 = R(d.alpha0), a1 = R(d.alpha1);
-  const P* ub = static_cast<const P*>(d.ub);
-  const P* f = static_cast<const P*>(d.f);
+  const auto ub = gptr<const P>(d.ub);
+  const auto f = gptr<const P>(d.f);
   double t[kGapTerms] = {0};
   const int cnt = (yb - ya) * W;
   for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
@@ -807,7 +829,7 @@ __global__ __launch_bounds__(kBlock) void k_tgv_dual(const SliceDev* sl, const i
     }
   }
   if (GAP) block_sum<4>(reinterpret_cast<double(&)[4]>(t), part + size_t(blockIdx.x) * kGapTerms);
-  if (ya == 0 && threadIdx.x == 0) d.ctl[CTL_ITER] += 1;
+  if (ya == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += 1;
 }
 
 // TGV dual restoration, step 1: d = div_sym q into tmp, per-item max |d|.
@@ -820,11 +842,11 @@ __global__ __launch_bounds__(kBlock) void k_tgv_restore1(const SliceDev* sl, con
   const int W = d.W, H = d.H;
   const size_t n = size_t(W) * H;
   const int ya = it.y, yb = min(H, ya + rpi);
-  const S* qxx = static_cast<const S*>(d.q);
-  const S* qyy = qxx + n;
-  const S* qxy = qyy + n;
-  P* dx = static_cast<P*>(d.tmp);
-  P* dy = dx + n;
+  const auto qxx = gptr<const S>(d.q);
+  const auto qyy = qxx + n;
+  const auto qxy = qyy + n;
+  const auto dx = gptr<P>(d.tmp);
+  const auto dy = dx + n;
   double m = 0;
   const int cnt = (yb - ya) * W;
   for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
@@ -851,9 +873,9 @@ __global__ __launch_bounds__(kBlock) void k_tgv_restore2(const SliceDev* sl, con
   const int W = d.W, H = d.H;
   const size_t n = size_t(W) * H;
   const int ya = it.y, yb = min(H, ya + rpi);
-  const P* dx = static_cast<const P*>(d.tmp);
-  const P* dy = dx + n;
-  const P* f = static_cast<const P*>(d.f);
+  const auto dx = gptr<const P>(d.tmp);
+  const auto dy = dx + n;
+  const auto f = gptr<const P>(d.f);
   double t[2] = {0, 0};
   const int cnt = (yb - ya) * W;
   for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
@@ -917,11 +939,11 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
   const int y = it.y / d.ngr, g = it.y - y * d.ngr;
   const int c0 = g * cb, ncb = min(cb, d.C - c0);
   RowLds<C> L(smem_raw, W, cb);
-  const C* gtw = static_cast<const C*>(d.planW.tw);
+  const auto gtw = gptr<const C>(d.planW.tw);
   for (int e = threadIdx.x; e < W; e += NT) L.tw[e] = gtw[e];
-  for (int e = threadIdx.x; e < nc; e += NT) L.cols[e] = d.cols[e];
-  const C* src = static_cast<const C*>(MODE == M_OP ? d.op_in : d.ub) + size_t(y) * W;
-  const C* S = static_cast<const C*>(d.S);
+  for (int e = threadIdx.x; e < nc; e += NT) L.cols[e] = gptr<const int>(d.cols)[e];
+  const auto src = gptr<const C>(MODE == M_OP ? d.op_in : d.ub) + size_t(y) * W;
+  const auto S = gptr<const C>(d.S);
   const int G = W * d.Hk;
   const C* Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : nullptr;  // coil c0, row y + y0
   const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
@@ -954,7 +976,7 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
     // pixel ranges of a multiple of 32, so no group carries all of it.
     const int chunk = ((W + d.ngr - 1) / d.ngr + 31) & ~31;
     const int xa = min(W, g * chunk), xe = min(W, xa + chunk);
-    const int k = d.ctl[CTL_ITER];
+    const int k = gptr<int>(d.ctl)[CTL_ITER];
     const R sigma = R(d.sigma);
     const R a0 = sched_alpha<R>(d, d.alpha0, d.alpha0_end, k);
     const R a1 = sched_alpha<R>(d, d.alpha1, d.alpha1_end, k);
@@ -973,7 +995,7 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
   }
   __syncthreads();
   C* res = fft_n<R, NT, false, NF>(L.a, L.b, ncb, d.planW, L.tw);
-  C* T1 = static_cast<C*>(d.T1) + (size_t(c0) * H + y) * nc;  // row y of coil c0
+  const auto T1 = gptr<C>(d.T1) + (size_t(c0) * H + y) * nc;  // row y of coil c0
   const int cstride = H * nc;
   for (int e = threadIdx.x; e < ncb * nc; e += NT) {
     const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
@@ -999,11 +1021,11 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   const int c = it.y / d.ngroups, g = it.y - c * d.ngroups;
   const int ks0 = g * cg, ncg = min(cg, nc - ks0);
   const R scale = R(1) / sqrt(R(W) * R(Hk));
-  const R* Mc = static_cast<const R*>(d.Mc);
+  const auto Mc = gptr<const R>(d.Mc);
   C* tw = reinterpret_cast<C*>(smem_raw);
   C* bufA = tw + Hk;
   C* bufB = bufA + lds_pad(cg * Hk);
-  const C* gtw = static_cast<const C*>(d.planH.tw);
+  const auto gtw = gptr<const C>(d.planH.tw);
   for (int e = threadIdx.x; e < Hk; e += NT) tw[e] = gtw[e];
   const float rHk = 1.0f / float(Hk), rncg = 1.0f / float(ncg);
   C* res;
@@ -1013,8 +1035,8 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   // they are loaded into registers here, together with the T1 gather, so that
   // their latency is not paid again after the forward FFT.
   const R sigma = R(d.sigma), lambda = R(d.lambda), inv_sl = rcp(R(1) + sigma * lambda);
-  const C* fk = static_cast<const C*>(d.fk) + (size_t(c) * nc + ks0) * Hk;
-  C* rk = static_cast<C*>(d.rk) + (size_t(c) * nc + ks0) * Hk;
+  const auto fk = gptr<const C>(d.fk) + (size_t(c) * nc + ks0) * Hk;
+  const auto rk = gptr<C>(d.rk) + (size_t(c) * nc + ks0) * Hk;
   const R* mk = Mc + size_t(ks0) * Hk;
   constexpr int KP = NF > 0 ? (kColPrefetchCols * NF + NT - 1) / NT : 1;
   C pf_f[KP], pf_r[KP];
@@ -1032,22 +1054,22 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
     }
   }
   if (CMODE != CM_ADJ) {
-    const C* T1 = static_cast<const C*>(d.T1) + size_t(c) * H * nc + ks0;  // coil c, column ks0
+    const auto T1 = gptr<const C>(d.T1) + size_t(c) * H * nc + ks0;  // coil c, column ks0
     for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
       const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
       const int fe = ishift_src(j, Hk);
       const int yy = fe - y0;
-      bufA[lp<NF>(cl * Hk + j)] = (yy >= 0 && yy < H) ? T1[yy * nc + cl] : pzero<C>();
+      bufA[lp<NF>(cl * Hk + j)] = (yy >= 0 && yy < H) ? C(T1[yy * nc + cl]) : pzero<C>();
     }
     __syncthreads();
     res = fft_n<R, NT, false, NF>(bufA, bufB, ncg, d.planH, tw);
     C* other = (res == bufA) ? bufB : bufA;
     if (CMODE == CM_FWD) {
-      C* out = static_cast<C*>(d.op_out);
+      const auto out = gptr<C>(d.op_out);
       for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
         const int cl = divn<NF>(e, Hk, rHk), kfe = e - cl * Hk;
         const R m = Mc[size_t(ks0 + cl) * Hk + kfe];
-        out[(size_t(c) * Hk + kfe) * W + d.cols[ks0 + cl]] = (m * scale) * res[lp<NF>(cl * Hk + fshift_src(kfe, Hk))];
+        out[(size_t(c) * Hk + kfe) * W + gptr<const int>(d.cols)[ks0 + cl]] = (m * scale) * res[lp<NF>(cl * Hk + fshift_src(kfe, Hk))];
       }
       return;
     }
@@ -1076,16 +1098,16 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
     __syncthreads();
     res = fft_n<R, NT, true, NF>(other, res, ncg, d.planH, tw);
   } else {
-    const C* in = static_cast<const C*>(d.op_in);
+    const auto in = gptr<const C>(d.op_in);
     for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
       const int cl = divn<NF>(e, Hk, rHk), kfe = e - cl * Hk;
       const R m = Mc[size_t(ks0 + cl) * Hk + kfe];
-      bufA[lp<NF>(cl * Hk + ishift_dst(kfe, Hk))] = m * in[(size_t(c) * Hk + kfe) * W + d.cols[ks0 + cl]];
+      bufA[lp<NF>(cl * Hk + ishift_dst(kfe, Hk))] = m * in[(size_t(c) * Hk + kfe) * W + gptr<const int>(d.cols)[ks0 + cl]];
     }
     __syncthreads();
     res = fft_n<R, NT, true, NF>(bufA, bufB, ncg, d.planH, tw);
   }
-  C* T2 = static_cast<C*>(d.T2) + size_t(c) * H * nc + ks0;
+  const auto T2 = gptr<C>(d.T2) + size_t(c) * H * nc + ks0;
   for (int e = threadIdx.x; e < ncg * H; e += NT) {
     const int yy = qdiv(e, ncg, rncg), cl = e - yy * ncg;
     T2[yy * nc + cl] = scale * res[lp<NF>(cl * Hk + fshift_src(yy + y0, Hk))];
@@ -1103,10 +1125,10 @@ __device__ __forceinline__ void mri_primal_pixel(const SliceDev& d, int x, int y
   using C = Cx<R>;
   const int W = d.W, H = d.H;
   const size_t n = size_t(W) * H, i = size_t(y) * W + x;
-  C* u = static_cast<C*>(d.u);
-  C* ub = static_cast<C*>(d.ub);
-  const C* px = static_cast<const C*>(d.p);
-  const C* py = px + n;
+  const auto u = gptr<C>(d.u);
+  const auto ub = gptr<C>(d.ub);
+  const auto px = gptr<const C>(d.p);
+  const auto py = px + n;
   const C uo = u[i];
   const C div = dmx<C>(px, x, i, W) + dmy<C>(py, y, i, W, H);
   const C un = uo + tau * (div - w);
@@ -1140,15 +1162,15 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   // the wide variant keeps two blocks per CU only without the LDS padding
   constexpr bool LP = NT != kRowAdjThreadsWide;
   RowLds<C> L(smem_raw, W, cb, LP);
-  const C* gtw = static_cast<const C*>(d.planW.tw);
+  const auto gtw = gptr<const C>(d.planW.tw);
   for (int e = threadIdx.x; e < W; e += NT) L.tw[e] = gtw[e];
   for (int e = threadIdx.x; e < (LP ? lds_pad(ncb * W) : ncb * W); e += NT) L.a[e] = pzero<C>();
-  for (int e = threadIdx.x; e < nc; e += NT) L.cols[e] = d.cols[e];
+  for (int e = threadIdx.x; e < nc; e += NT) L.cols[e] = gptr<const int>(d.cols)[e];
   __syncthreads();
   const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
   // sensitivities of the group's coils on row y + y0: with a compile-time
   // length loaded into registers before the T2 gather (one latency for both)
-  const C* S = static_cast<const C*>(d.S);
+  const auto S = gptr<const C>(d.S);
   const int G = W * d.Hk;
   const C* Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : nullptr;
   C pS[kPrefetch];
@@ -1177,15 +1199,15 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
     const int x = threadIdx.x;
     if (x < W) {
       const size_t n = size_t(W) * H, i = size_t(y) * W + x;
-      const C* px = static_cast<const C*>(d.p);
-      e_uo = static_cast<const C*>(d.u)[i];
+      const auto px = gptr<const C>(d.p);
+      e_uo = gptr<const C>(d.u)[i];
       e_div = dmx<C>(px, x, i, W) + dmy<C>(px + n, y, i, W, H);
     } else if (MODE == M_TGV && x < 2 * W) {
       const double rv = vstep_pixel<C, C>(d, x - W, y, tau, theta);
       if (GAP) t[1] += rv;
     }
   }
-  const C* T2 = static_cast<const C*>(d.T2) + (size_t(c0) * H + y) * nc;  // row y of coil c0
+  const auto T2 = gptr<const C>(d.T2) + (size_t(c0) * H + y) * nc;  // row y of coil c0
   const int cstride = H * nc;
   for (int e = threadIdx.x; e < ncb * nc; e += NT) {
     const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
@@ -1212,7 +1234,7 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   }
   __syncthreads();
   if (d.ngr_a > 1) {
-    C* wp = static_cast<C*>(d.wpart) + (size_t(g) * H + y) * W;
+    const auto wp = gptr<C>(d.wpart) + (size_t(g) * H + y) * W;
     for (int x = threadIdx.x; x < W; x += NT) {
       C acc = other[lp<NF, LP>(x)];
       for (int cl = 1; cl < ncb; ++cl) acc = acc + other[lp<NF, LP>(cl * W + x)];
@@ -1227,23 +1249,23 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
       for (int cl = 1; cl < ncb; ++cl) w = w + other[lp<NF, LP>(cl * W + x)];
       const size_t i = size_t(y) * W + x;
       const C un = e_uo + tau * (e_div - w);
-      static_cast<C*>(d.u)[i] = un;
-      static_cast<C*>(d.ub)[i] = (R(1) + theta) * un - theta * e_uo;
+      gptr<C>(d.u)[i] = un;
+      gptr<C>(d.ub)[i] = (R(1) + theta) * un - theta * e_uo;
       if (GAP) t[0] += dabs2(w - e_div);
     }
   } else {
     for (int x = threadIdx.x; x < W; x += NT) {
       C w = other[lp<NF, LP>(x)];
       for (int cl = 1; cl < ncb; ++cl) w = w + other[lp<NF, LP>(cl * W + x)];
-      if (MODE == M_OP) static_cast<C*>(d.op_out)[size_t(y) * W + x] = w;
+      if (MODE == M_OP) gptr<C>(d.op_out)[size_t(y) * W + x] = w;
       else mri_primal_pixel<R, MODE, GAP>(d, x, y, w, tau, theta, t);
     }
   }
   if (MODE == M_OP) return;
   if (GAP) block_sum<2, NT>(t, part_rows + size_t(d.row_beg + y) * kGapTerms);
   if (y == 0 && threadIdx.x == 0) {
-    d.ctl[CTL_ITER] += 1;
-    d.ctl[CTL_RUPD] += 1;
+    gptr<int>(d.ctl)[CTL_ITER] += 1;
+    gptr<int>(d.ctl)[CTL_RUPD] += 1;
   }
 }
 
@@ -1264,7 +1286,7 @@ __global__ __launch_bounds__(1024) void k_mri_primal(const SliceDev* sl, const i
   const int W = d.W, H = d.H, ngr = d.ngr_a;
   const size_t n = size_t(W) * H;
   const int ya = it.y, yb = min(H, ya + rpi);
-  const C* wp = static_cast<const C*>(d.wpart);
+  const auto wp = gptr<const C>(d.wpart);
   const R tau = R(d.tau), theta = R(d.theta);
   double t[2] = {0, 0};
   const int cnt = (yb - ya) * W;
@@ -1274,14 +1296,14 @@ __global__ __launch_bounds__(1024) void k_mri_primal(const SliceDev* sl, const i
     const size_t i = size_t(y) * W + x;
     C w = wp[i];
     for (int g = 1; g < ngr; ++g) w = w + wp[size_t(g) * n + i];
-    if (MODE == M_OP) static_cast<C*>(d.op_out)[i] = w;
+    if (MODE == M_OP) gptr<C>(d.op_out)[i] = w;
     else mri_primal_pixel<R, MODE, GAP>(d, x, y, w, tau, theta, t);
   }
   if (MODE == M_OP) return;
   if (GAP) block_sum_dyn<2>(t, part + size_t(li) * kGapTerms);
   if (ya == 0 && threadIdx.x == 0) {
-    d.ctl[CTL_ITER] += 1;
-    d.ctl[CTL_RUPD] += 1;
+    gptr<int>(d.ctl)[CTL_ITER] += 1;
+    gptr<int>(d.ctl)[CTL_RUPD] += 1;
   }
 }
 
@@ -1355,11 +1377,11 @@ __global__ __launch_bounds__(kBlock) void k_gap_reduce(const SliceDev* sl, int k
   } else {
     // Partial gap on the ball of radius (rho_u, rho_v) around (ubar, vbar):
     // sum of Fenchel-Young gaps plus rho * stationarity residuals.
-    const int k = d.ctl[CTL_ITER] - 1;
+    const int k = gptr<int>(d.ctl)[CTL_ITER] - 1;
     const bool tgv = kind != K_TV_OP;
     const double a0 = tgv ? double(sched_alpha<R>(d, d.alpha0, d.alpha0_end, k)) : 0.0;
     const double a1 = tgv ? double(sched_alpha<R>(d, d.alpha1, d.alpha1_end, k)) : 1.0;
-    const double bk = pow(1.0 + d.sigma * lam, -double(d.ctl[CTL_RUPD]));
+    const double bk = pow(1.0 + d.sigma * lam, -double(gptr<int>(d.ctl)[CTL_RUPD]));
     const double preg = a1 * tot[0] + a0 * tot[1];
     P = (tot[3] + d.F0) / (2.0 * lam) + preg;
     const double fy = (tot[4] + d.F0 * bk * bk) / (2.0 * lam) + preg - tot[2];
@@ -1377,10 +1399,10 @@ __global__ __launch_bounds__(kBlock) void k_gap_reduce(const SliceDev* sl, int k
   d.stats[ST_C0] = c0;
   d.stats[ST_C1] = c1;
   d.stats[ST_C2] = c2;
-  d.ctl[CTL_NGAP] += 1;
+  gptr<int>(d.ctl)[CTL_NGAP] += 1;
   if (d.gap_tol > 0 && rel <= d.gap_tol) {
-    d.ctl[CTL_ACTIVE] = 0;
-    d.ctl[CTL_DONE_AT] = d.ctl[CTL_ITER];
+    gptr<int>(d.ctl)[CTL_ACTIVE] = 0;
+    gptr<int>(d.ctl)[CTL_DONE_AT] = gptr<int>(d.ctl)[CTL_ITER];
   }
 }
 
@@ -1398,12 +1420,12 @@ __global__ __launch_bounds__(kBlock) void k_compact(const SliceDev* sl, const in
   using C = Cx<R>;
   const int W = d.W, Hk = d.Hk, nc = d.ncols;
   const int c = it.y / Hk, kfe = it.y - c * Hk;
-  const C* f = static_cast<const C*>(f_full) + (size_t(c) * Hk + kfe) * W;
-  C* fk = static_cast<C*>(d.fk);
-  C* rk = static_cast<C*>(d.rk);
+  const auto f = gptr<const C>(f_full) + (size_t(c) * Hk + kfe) * W;
+  const auto fk = gptr<C>(d.fk);
+  const auto rk = gptr<C>(d.rk);
   double t[1] = {0};
   for (int kx = threadIdx.x; kx < W; kx += kBlock) {
-    const int ks = d.colmap[kx];
+    const int ks = gptr<const int>(d.colmap)[kx];
     const C v = f[kx];
     if (ks >= 0) {
       const size_t idx = (size_t(c) * nc + ks) * Hk + kfe;
