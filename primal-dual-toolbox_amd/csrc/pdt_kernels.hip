@@ -586,15 +586,22 @@ __device__ __forceinline__ int ishift_src(int j, int n) {
 // ---------------------------------------------------------------------------
 // Stencil helpers on plane pointers (tgv_kernels.cuh:25-135)
 // ---------------------------------------------------------------------------
+// Both neighbours are loaded unconditionally (the first row / column reads
+// its own element twice) and the boundary cases selected afterwards, so the
+// loads issue together instead of behind branches.
 template <class P, class S>
 __device__ __forceinline__ P dmx(const S* a, int x, size_t i, int W) {
   using L = Ld<P, Plain<S>>;
-  return (x > 0) ? ((x < W - 1) ? L::get(a, i) - L::get(a, i - 1) : -L::get(a, i - 1)) : L::get(a, i);
+  const P c = L::get(a, i), m = L::get(a, x > 0 ? i - 1 : i);
+  const P d = (x < W - 1) ? c - m : -m;
+  return x > 0 ? d : c;
 }
 template <class P, class S>
 __device__ __forceinline__ P dmy(const S* a, int y, size_t i, int W, int H) {
   using L = Ld<P, Plain<S>>;
-  return (y > 0) ? ((y < H - 1) ? L::get(a, i) - L::get(a, i - W) : -L::get(a, i - W)) : L::get(a, i);
+  const P c = L::get(a, i), m = L::get(a, y > 0 ? i - W : i);
+  const P d = (y < H - 1) ? c - m : -m;
+  return y > 0 ? d : c;
 }
 
 // 1 / x: the hardware reciprocal (within 1 ulp) in single precision, a
@@ -921,6 +928,35 @@ struct RowLds {
   }
 };
 
+// A short global array (twiddles, the column list) staged through registers:
+// with a compile-time length the loads are issued at the start of a kernel
+// together with its other loads and written to LDS after the first wait,
+// instead of a load-wait-write loop that costs a memory latency of its own.
+template <class T, int NF, int NT>
+struct Staged {
+  static constexpr int K = NF > 0 ? (NF + NT - 1) / NT : 1;
+  T v[K];
+  template <class A>
+  __device__ __forceinline__ void load(A* src, int n) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < n) v[k] = src[e];
+    }
+  }
+  __device__ __forceinline__ void store(T* dst, int n) const {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < n) dst[e] = v[k];
+    }
+  }
+};
+template <class T, int NF, int NT, class A>
+__device__ __forceinline__ void copy_to_lds(T* dst, A* src, int n) {
+  for (int e = threadIdx.x; e < n; e += NT) dst[e] = src[e];
+}
+
 // Row pass, forward.  Item = (slice, y * ngr + g): coil group g of image row
 // y.  The groups of row y also perform the dual step of row y (TV or TGV),
 // each on its own range of pixels.  Then for
@@ -940,12 +976,20 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
   const int c0 = g * cb, ncb = min(cb, d.C - c0);
   RowLds<C> L(smem_raw, W, cb);
   const auto gtw = gptr<const C>(d.planW.tw);
-  for (int e = threadIdx.x; e < W; e += NT) L.tw[e] = gtw[e];
-  for (int e = threadIdx.x; e < nc; e += NT) L.cols[e] = gptr<const int>(d.cols)[e];
+  const auto gcols = gptr<const int>(d.cols);
+  Staged<C, NF, NT> s_tw;
+  Staged<int, NF, NT> s_cols;
+  if (NF > 0) {
+    s_tw.load(gtw, W);
+    s_cols.load(gcols, nc);
+  } else {
+    copy_to_lds<C, NF, NT>(L.tw, gtw, W);
+    copy_to_lds<int, NF, NT>(L.cols, gcols, nc);
+  }
   const auto src = gptr<const C>(MODE == M_OP ? d.op_in : d.ub) + size_t(y) * W;
   const auto S = gptr<const C>(d.S);
   const int G = W * d.Hk;
-  const C* Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : nullptr;  // coil c0, row y + y0
+  const auto Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : S;  // coil c0, row y + y0
   const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
   // With a compile-time length and at most kPrefetch elements per thread the
   // row and sensitivity values are loaded into registers first and combined
@@ -982,6 +1026,10 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
     const R a1 = sched_alpha<R>(d, d.alpha1, d.alpha1_end, k);
     const R ia0 = rcp(a0), ia1 = rcp(a1);
     for (int x = xa + threadIdx.x; x < xe; x += NT) dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, ia0, ia1, t);
+  }
+  if (NF > 0) {
+    s_tw.store(L.tw, W);
+    s_cols.store(L.cols, nc);
   }
   if (pf) {
 #pragma unroll
@@ -1026,7 +1074,9 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   C* bufA = tw + Hk;
   C* bufB = bufA + lds_pad(cg * Hk);
   const auto gtw = gptr<const C>(d.planH.tw);
-  for (int e = threadIdx.x; e < Hk; e += NT) tw[e] = gtw[e];
+  Staged<C, NF, NT> s_tw;
+  if (NF > 0) s_tw.load(gtw, Hk);
+  else copy_to_lds<C, NF, NT>(tw, gtw, Hk);
   const float rHk = 1.0f / float(Hk), rncg = 1.0f / float(ncg);
   C* res;
   double t[2] = {0, 0};
@@ -1037,7 +1087,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   const R sigma = R(d.sigma), lambda = R(d.lambda), inv_sl = rcp(R(1) + sigma * lambda);
   const auto fk = gptr<const C>(d.fk) + (size_t(c) * nc + ks0) * Hk;
   const auto rk = gptr<C>(d.rk) + (size_t(c) * nc + ks0) * Hk;
-  const R* mk = Mc + size_t(ks0) * Hk;
+  const auto mk = Mc + size_t(ks0) * Hk;
   constexpr int KP = NF > 0 ? (kColPrefetchCols * NF + NT - 1) / NT : 1;
   C pf_f[KP], pf_r[KP];
   R pf_m[KP];
@@ -1055,12 +1105,36 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   }
   if (CMODE != CM_ADJ) {
     const auto T1 = gptr<const C>(d.T1) + size_t(c) * H * nc + ks0;  // coil c, column ks0
-    for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
-      const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
-      const int fe = ishift_src(j, Hk);
-      const int yy = fe - y0;
-      bufA[lp<NF>(cl * Hk + j)] = (yy >= 0 && yy < H) ? C(T1[yy * nc + cl]) : pzero<C>();
+    // the gather of the zero-padded, ifftshifted columns: into registers with
+    // the prefetch (one latency for both), else straight into LDS
+    C pf_t[KP];
+    if (pf) {
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int e = threadIdx.x + k * NT;
+        if (e < ncg * Hk) {
+          const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
+          const int yy = ishift_src(j, Hk) - y0;
+          pf_t[k] = (yy >= 0 && yy < H) ? C(T1[yy * nc + cl]) : pzero<C>();
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int e = threadIdx.x + k * NT;
+        if (e < ncg * Hk) {
+          const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
+          bufA[lp<NF>(cl * Hk + j)] = pf_t[k];
+        }
+      }
+    } else {
+      for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
+        const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
+        const int fe = ishift_src(j, Hk);
+        const int yy = fe - y0;
+        bufA[lp<NF>(cl * Hk + j)] = (yy >= 0 && yy < H) ? C(T1[yy * nc + cl]) : pzero<C>();
+      }
     }
+    if (NF > 0) s_tw.store(tw, Hk);
     __syncthreads();
     res = fft_n<R, NT, false, NF>(bufA, bufB, ncg, d.planH, tw);
     C* other = (res == bufA) ? bufB : bufA;
@@ -1098,6 +1172,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
     __syncthreads();
     res = fft_n<R, NT, true, NF>(other, res, ncg, d.planH, tw);
   } else {
+    if (NF > 0) s_tw.store(tw, Hk);
     const auto in = gptr<const C>(d.op_in);
     for (int e = threadIdx.x; e < ncg * Hk; e += NT) {
       const int cl = divn<NF>(e, Hk, rHk), kfe = e - cl * Hk;
@@ -1163,27 +1238,51 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   constexpr bool LP = NT != kRowAdjThreadsWide;
   RowLds<C> L(smem_raw, W, cb, LP);
   const auto gtw = gptr<const C>(d.planW.tw);
-  for (int e = threadIdx.x; e < W; e += NT) L.tw[e] = gtw[e];
+  const auto gcols = gptr<const int>(d.cols);
+  Staged<C, NF, NT> s_tw;
+  if (NF > 0) s_tw.load(gtw, W);
+  else copy_to_lds<C, NF, NT>(L.tw, gtw, W);
   for (int e = threadIdx.x; e < (LP ? lds_pad(ncb * W) : ncb * W); e += NT) L.a[e] = pzero<C>();
-  for (int e = threadIdx.x; e < nc; e += NT) L.cols[e] = gptr<const int>(d.cols)[e];
-  __syncthreads();
   const float rW = 1.0f / float(W), rnc = 1.0f / float(max(nc, 1));
   // sensitivities of the group's coils on row y + y0: with a compile-time
   // length loaded into registers before the T2 gather (one latency for both)
   const auto S = gptr<const C>(d.S);
   const int G = W * d.Hk;
-  const C* Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : nullptr;
-  C pS[kPrefetch];
-  const bool pf = NF > 0 && Sg && ncb * W <= kPrefetch * NT;
+  const auto Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : S;
+  // register slots per thread: the 1024-thread variant must stay within 64
+  // VGPRs (two blocks per CU), so it holds fewer
+  constexpr int PS = NT >= 1024 ? 6 : kPrefetch, PT = NT >= 1024 ? 2 : 4;
+  C pS[PS];
+  const bool pf = NF > 0 && Sg && ncb * W <= PS * NT;
   if (pf) {
 #pragma unroll
-    for (int k = 0; k < kPrefetch; ++k) {
+    for (int k = 0; k < PS; ++k) {
       const int e = threadIdx.x + k * NT;
       if (e < ncb * W) {
         const int cl = divn<NF>(e, W, rW), x = e - cl * W;
         pS[k] = Sg[cl * G + x];
       }
     }
+  }
+  // the T2 gather (values and their PE columns) into registers with a
+  // compile-time length, issued with the sensitivity loads
+  const auto T2 = gptr<const C>(d.T2) + (size_t(c0) * H + y) * nc;  // row y of coil c0
+  const int cstride = H * nc;
+  C pT[PT];
+  int pK[PT];
+  const bool pft = NF > 0 && ncb * nc <= PT * NT;
+  if (pft) {
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < ncb * nc) {
+        const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
+        pT[k] = T2[cl * cstride + ks];
+        pK[k] = gcols[ks];
+      }
+    }
+  } else {
+    copy_to_lds<int, NF, NT>(L.cols, gcols, nc);
   }
   // With all coils of the row in one block of at least 2 W threads, the
   // parts of the primal step that do not need A^* r run before the FFT, so
@@ -1207,18 +1306,29 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
       if (GAP) t[1] += rv;
     }
   }
-  const auto T2 = gptr<const C>(d.T2) + (size_t(c0) * H + y) * nc;  // row y of coil c0
-  const int cstride = H * nc;
-  for (int e = threadIdx.x; e < ncb * nc; e += NT) {
-    const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
-    L.a[lp<NF, LP>(cl * W + ishift_dst(L.cols[ks], W))] = T2[cl * cstride + ks];
+  if (NF > 0) s_tw.store(L.tw, W);
+  __syncthreads();  // zero fill, twiddles, column list
+  if (pft) {
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < ncb * nc) {
+        const int cl = qdiv(e, nc, rnc);
+        L.a[lp<NF, LP>(cl * W + ishift_dst(pK[k], W))] = pT[k];
+      }
+    }
+  } else {
+    for (int e = threadIdx.x; e < ncb * nc; e += NT) {
+      const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
+      L.a[lp<NF, LP>(cl * W + ishift_dst(L.cols[ks], W))] = T2[cl * cstride + ks];
+    }
   }
   __syncthreads();
   C* res = fft_n<R, NT, true, NF, LP>(L.a, L.b, ncb, d.planW, L.tw);
   C* other = (res == L.a) ? L.b : L.a;
   if (pf) {
 #pragma unroll
-    for (int k = 0; k < kPrefetch; ++k) {
+    for (int k = 0; k < PS; ++k) {
       const int e = threadIdx.x + k * NT;
       if (e < ncb * W) {
         const int cl = divn<NF>(e, W, rW), x = e - cl * W;
