@@ -33,6 +33,7 @@ enum Kind { K_TV = 0, K_TGV = 1, K_TGV_MRI = 2, K_TV_OP = 3, K_TGV_OP = 4 };
 // tws = n / (p r) (twiddle stride), and float reciprocals of nr and p for
 // division-free index arithmetic.
 struct FftPlanD {
+  const void* tw;  // Cx<R>[n]: exp(-2 pi i m / n)
   int n;
   int ns;
   int r[kMaxStages];
@@ -41,7 +42,6 @@ struct FftPlanD {
   int tws[kMaxStages];
   float rnr[kMaxStages];
   float rp[kMaxStages];
-  const void* tw;  // Cx<R>[n]: exp(-2 pi i m / n)
 };
 
 // Control words of a slice (device memory).
@@ -49,7 +49,35 @@ enum Ctl { CTL_ACTIVE = 0, CTL_ITER = 1, CTL_RUPD = 2, CTL_DONE_AT = 3, CTL_NGAP
 // Statistics doubles of a slice (device memory), written by the gap reduction.
 enum Stat { ST_P = 0, ST_D = 1, ST_GAP = 2, ST_REL = 3, ST_C0 = 4, ST_C1 = 5, ST_C2 = 6, ST_WORDS = 8 };
 
+// Field order: what the iteration kernels read first (control words, device
+// pointers, sizes, scalar parameters) sits in the first few cache lines, so
+// a block's scalar loads of its descriptor touch few lines; the FFT plans,
+// read in full only by the runtime-planned kernels, come last.
 struct SliceDev {
+  // control and results
+  int* ctl;
+  double* stats;
+  // k-space (operator solvers)
+  const int* cols;    // ks -> kx
+  const int* colmap;  // kx -> ks or -1
+  void* fk;           // compact data [c][ks][kfe]
+  void* rk;           // compact dual r [c][ks][kfe]
+  const void* S;      // coil sensitivities {W, Hk, C}; null for the sampling op
+  const void* Mc;     // compact mask [ks][kfe] (real)
+  void* T1;           // row-pass output [c][y][ks]
+  void* T2;           // column-pass output [c][y][ks]
+  void* wpart;        // per-coil-group partial adjoints [g][y][x]
+  const void* op_in;  // operator mode input
+  void* op_out;       // operator mode output
+  // image fields, planes of W*H pixels
+  void* u;
+  void* ub;
+  void* v;   // 2 planes (x, y)
+  void* vb;  // 2 planes
+  void* p;   // 2 planes
+  void* q;   // 3 planes (xx, yy, xy)
+  const void* f;  // denoising data
+  void* tmp;      // 2 planes: TGV dual-restoration scratch
   // geometry
   int W, H;       // image {W (x, contiguous), H (y)}
   int Hk, y0, C;  // k-space FE length, FE offset of the image window, coils
@@ -68,34 +96,10 @@ struct SliceDev {
   double radius_u, radius_v;      // partial-gap ball radii
   double F0;                      // ||f||^2 on columns without samples
   double gap_tol;
-  // image fields, planes of W*H pixels
-  void* u;
-  void* ub;
-  void* v;   // 2 planes (x, y)
-  void* vb;  // 2 planes
-  void* p;   // 2 planes
-  void* q;   // 3 planes (xx, yy, xy)
-  const void* f;  // denoising data
-  void* tmp;      // 2 planes: TGV dual-restoration scratch
-  // k-space (operator solvers)
-  void* fk;         // compact data [c][ks][kfe]
-  void* rk;         // compact dual r [c][ks][kfe]
-  const void* S;    // coil sensitivities {W, Hk, C}; null for the sampling op
-  const void* Mc;   // compact mask [ks][kfe] (real)
-  const int* cols;    // ks -> kx
-  const int* colmap;  // kx -> ks or -1
-  void* T1;         // row-pass output [c][y][ks]
-  void* T2;         // column-pass output [c][y][ks]
-  void* wpart;      // per-coil-group partial adjoints [g][y][x]
-  const void* op_in;  // operator mode input
-  void* op_out;       // operator mode output
-  FftPlanD planW, planH;
-  // control and results
-  int* ctl;
-  double* stats;
   // partial-sum ranges of this slice in the work lists: image rows,
   // (row, coil group) items and column items
   int row_beg, row_end, rg_beg, rg_end, col_beg, col_end;
+  FftPlanD planW, planH;
 };
 
 }  // namespace pdt

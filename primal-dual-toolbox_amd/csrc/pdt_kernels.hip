@@ -398,9 +398,11 @@ __device__ __forceinline__ int divn(int a, int n, float rn) {
 
 // Transforms nseq sequences of length pl.n stored back to back in a; b is
 // scratch of the same size; tw is the twiddle table (LDS or global).
-// Returns the buffer that holds the result.  Ends with a barrier.
+// Returns the buffer that holds the result.  Ends with a barrier.  Inlined,
+// so that a and b stay LDS pointers (a call would pass generic ones and turn
+// every access into a flat one) and the caller's registers need no saving.
 template <class R, int NT>
-__device__ Cx<R>* lds_fft(Cx<R>* a, Cx<R>* b, int nseq, const FftPlanD& pl, const Cx<R>* tw, bool inv) {
+__device__ __forceinline__ Cx<R>* lds_fft(Cx<R>* a, Cx<R>* b, int nseq, const FftPlanD& pl, const Cx<R>* tw, bool inv) {
   const int n = pl.n;
   for (int s = 0; s < pl.ns; ++s) {
     const int r = pl.r[s], nr = pl.nr[s], p = pl.p[s], tws = pl.tws[s];
@@ -969,7 +971,6 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
   const int li = xcd_item(blockIdx.x, nitems);
   const int2 it = items[li];
   const SliceDev& d = sl[it.x];
-  if (!slice_active(d)) return;
   using C = Cx<R>;
   const int W = NF > 0 ? NF : d.W, H = d.H, nc = d.ncols;
   const int y = it.y / d.ngr, g = it.y - y * d.ngr;
@@ -1014,6 +1015,9 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2*
       L.a[lp<NF>(cl * W + ishift_dst(x, W))] = z;
     }
   }
+  // A finished slice's blocks leave before their first store; the check comes
+  // after the loads above are issued, so its control-word load overlaps them.
+  if (!slice_active(d)) return;
   double t[kGapTerms] = {0};
   if (MODE != M_OP) {
     // The dual step of row y is shared out over the row's coil groups in
@@ -1063,7 +1067,6 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   const int li = xcd_item(blockIdx.x, nitems);
   const int2 it = items[li];
   const SliceDev& d = sl[it.x];
-  if (!slice_active(d)) return;
   using C = Cx<R>;
   const int W = d.W, H = d.H, Hk = NF > 0 ? NF : d.Hk, y0 = d.y0, nc = d.ncols;
   const int c = it.y / d.ngroups, g = it.y - c * d.ngroups;
@@ -1103,6 +1106,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
       }
     }
   }
+  if (!slice_active(d)) return;  // after the prefetch is issued, before any store
   if (CMODE != CM_ADJ) {
     const auto T1 = gptr<const C>(d.T1) + size_t(c) * H * nc + ks0;  // coil c, column ks0
     // the gather of the zero-padded, ifftshifted columns: into registers with
@@ -1229,7 +1233,6 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   const int li = xcd_item(blockIdx.x, nitems);
   const int2 it = items[li];
   const SliceDev& d = sl[it.x];
-  if (!slice_active(d)) return;
   using C = Cx<R>;
   const int W = NF > 0 ? NF : d.W, H = d.H, nc = d.ncols;
   const int y = it.y / d.ngr_a, g = it.y - y * d.ngr_a;
@@ -1284,6 +1287,7 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   } else {
     copy_to_lds<int, NF, NT>(L.cols, gcols, nc);
   }
+  if (!slice_active(d)) return;  // after the loads above are issued, before any store
   // With all coils of the row in one block of at least 2 W threads, the
   // parts of the primal step that do not need A^* r run before the FFT, so
   // their loads overlap those of S and T2: threads x < W load u(x, y) and
