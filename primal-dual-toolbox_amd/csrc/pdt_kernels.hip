@@ -262,6 +262,47 @@ template <class R> struct Dft<R, 8> {
   }
 };
 
+// Radices 6 and 10: even/odd split into two DFTs of half length m,
+// X[k] = E[k] + w^k O[k], X[k + m] = E[k] - w^k O[k], w = exp(s i pi / m).
+template <class R> struct Dft<R, 6> {
+  static __device__ __forceinline__ void run(Cx<R>* x, R s) {
+    Cx<R> e[3] = {x[0], x[2], x[4]};
+    Cx<R> o[3] = {x[1], x[3], x[5]};
+    Dft<R, 3>::run(e, s);
+    Dft<R, 3>::run(o, s);
+    const R h = R(0.86602540378443864676);  // sin(pi / 3)
+    const Cx<R> t1 = o[1] * Cx<R>{R(0.5), s * h}, t2 = o[2] * Cx<R>{R(-0.5), s * h};
+    x[0] = e[0] + o[0];
+    x[3] = e[0] - o[0];
+    x[1] = e[1] + t1;
+    x[4] = e[1] - t1;
+    x[2] = e[2] + t2;
+    x[5] = e[2] - t2;
+  }
+};
+template <class R> struct Dft<R, 10> {
+  static __device__ __forceinline__ void run(Cx<R>* x, R s) {
+    Cx<R> e[5] = {x[0], x[2], x[4], x[6], x[8]};
+    Cx<R> o[5] = {x[1], x[3], x[5], x[7], x[9]};
+    Dft<R, 5>::run(e, s);
+    Dft<R, 5>::run(o, s);
+    // cos and sin of k pi / 5, k = 1, 2
+    const R c1 = R(0.80901699437494742410), c2 = R(0.30901699437494742410);
+    const R s1 = R(0.58778525229247312917), s2 = R(0.95105651629515357212);
+    Cx<R> t[5];
+    t[0] = o[0];
+    t[1] = o[1] * Cx<R>{c1, s * s1};
+    t[2] = o[2] * Cx<R>{c2, s * s2};
+    t[3] = o[3] * Cx<R>{-c2, s * s2};
+    t[4] = o[4] * Cx<R>{-c1, s * s1};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      x[k] = e[k] + t[k];
+      x[k + 5] = e[k] - t[k];
+    }
+  }
+};
+
 template <class R, int RR>
 __device__ __forceinline__ void butterfly(const Cx<R>* in, Cx<R>* out, int nr, int p,
                                           const Cx<R>* tw, int ts, bool inv) {
@@ -434,11 +475,16 @@ __device__ __forceinline__ Cx<R>* lds_fft(Cx<R>* a, Cx<R>* b, int nseq, const Ff
 }
 
 // ---------------------------------------------------------------------------
-// Compile-time-planned Stockham FFT for the common transform lengths.  Same
-// factorisation and stage formula as lds_fft, with radix, stride and twiddle
-// steps constant-folded; lengths without a specialisation use lds_fft.
+// Compile-time-planned Stockham FFT for the common transform lengths.  The
+// stage formula of lds_fft, with radix, stride and twiddle steps
+// constant-folded; lengths without a specialisation use lds_fft.
 // ---------------------------------------------------------------------------
 constexpr int ct_radix(int n, int s) {
+  // 384 and 640: a last stage of radix 6 or 10 instead of a radix-2 stage
+  // followed by a radix-3 or radix-5 one (the first stage stays radix 8,
+  // as the padded LDS layout requires).
+  if (n == 384) return s < 2 ? 8 : s == 2 ? 6 : 1;
+  if (n == 640) return s < 2 ? 8 : s == 2 ? 10 : 1;
   int k = 0;
   while (n % 8 == 0 && n > 8) {
     if (k == s) return 8;
@@ -494,7 +540,7 @@ __device__ __forceinline__ int lp(int e) {
 template <class R, int RR, bool TW>
 __device__ __forceinline__ void butterfly_ct(const Cx<R>* in, Cx<R>* out, int ts, const Cx<R>* tw, bool inv,
                                              int nr, int p) {
-  if constexpr (RR == 2 || RR == 3 || RR == 4 || RR == 5 || RR == 8) {
+  if constexpr (RR == 2 || RR == 3 || RR == 4 || RR == 5 || RR == 6 || RR == 8 || RR == 10) {
     Cx<R> x[RR];
     x[0] = in[0];
 #pragma unroll
@@ -503,7 +549,7 @@ __device__ __forceinline__ void butterfly_ct(const Cx<R>* in, Cx<R>* out, int ts
 #pragma unroll
     for (int q = 0; q < RR; ++q) out[q * p] = x[q];
   } else {
-    static_assert(RR % 2 == 1, "even radices are factored into 2, 4 and 8");
+    static_assert(RR % 2 == 1, "even radices are 2, 4, 6, 8 and 10");
     butterfly_odd<R, RR>(in, out, nr, p, tw, ts, inv);
   }
 }
@@ -519,7 +565,7 @@ __device__ __forceinline__ Cx<R>* fft_ct_stage(Cx<R>* a, Cx<R>* b, int nseq, con
     constexpr bool PAD = LP && ct_pad_ok(N);
     constexpr int NP = PAD ? N + N / 8 : N, nrp = PAD ? nr + nr / 8 : nr, pp = PAD && P > 1 ? P + P / 8 : P;
     const int total = nseq * nr;
-    if constexpr (r > 8) {
+    if constexpr (r > 8 && r % 2 == 1) {
       // Odd prime radix: a long chain of products per butterfly and few
       // butterflies per block, so the output pairs of each butterfly are
       // split over G waves (G = 4, 2 or 1, as many as keep every butterfly
