@@ -528,6 +528,11 @@ struct pdt_solver {
   std::unique_ptr<pdt_array> u, v;  // result handles (u: {W,H}, v: {W,H,2})
   DevBuf ub, vb, p, q, tmp;
   DevBuf fk, rk, T1, T2, wpart;
+  // set-up buffers kept across solves: descriptor and item list of the data
+  // compaction, its partial sums, A^*f, and two reduction results
+  DevBuf prep_sl, prep_items, prep_part, atf, red;
+  SliceDev prep_d{};  // host source of prep_sl (alive until its copy completes)
+  int prep_C = 0, prep_Hk = 0;
   int Hk = 0, C = 0, ncols = 0;
   double F0 = 0, atf_norm = 0;
   int rupd = 0;  // r updates since set_noisy_data
@@ -621,53 +626,52 @@ void prepare_op_data(pdt_solver* s) {
   s->Hk = op->Hk;
   s->C = op->C;
   s->ncols = op->ncols;
-  s->fk.alloc(std::max<size_t>(1, size_t(op->C) * op->ncols * op->Hk * ce));
-  s->rk.alloc(std::max<size_t>(1, size_t(op->C) * op->ncols * op->Hk * ce));
-  s->T1.alloc(std::max<size_t>(1, size_t(op->C) * op->H * op->ncols * ce));
-  s->T2.alloc(std::max<size_t>(1, size_t(op->C) * op->H * op->ncols * ce));
+  // buffers are reused while large enough: no device allocation (and no
+  // synchronising free) per solve, and stable addresses for the cached engine
+  s->fk.ensure(std::max<size_t>(1, size_t(op->C) * op->ncols * op->Hk * ce));
+  s->rk.ensure(std::max<size_t>(1, size_t(op->C) * op->ncols * op->Hk * ce));
+  s->T1.ensure(std::max<size_t>(1, size_t(op->C) * op->H * op->ncols * ce));
+  s->T2.ensure(std::max<size_t>(1, size_t(op->C) * op->H * op->ncols * ce));
   // compact gather + F0
-  SliceDev d = op->geometry();
+  SliceDev& d = s->prep_d;
+  d = op->geometry();
   d.fk = s->fk.p;
   d.rk = s->rk.p;
-  DevBuf dsl(sizeof(SliceDev));
-  HIPC(hipMemcpy(dsl.p, &d, sizeof(d), hipMemcpyHostToDevice));
-  std::vector<int2> items;
-  for (int c = 0; c < op->C; ++c)
-    for (int kfe = 0; kfe < op->Hk; ++kfe) items.push_back(make_int2(0, c * op->Hk + kfe));
-  DevBuf ditems(items.size() * sizeof(int2));
-  HIPC(hipMemcpy(ditems.p, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
-  DevBuf part(items.size() * kGapTerms * sizeof(double));
+  s->prep_sl.ensure(sizeof(SliceDev));
+  HIPC(hipMemcpyAsync(s->prep_sl.p, &d, sizeof(d), hipMemcpyHostToDevice, s->stream));
+  const int nitems = op->C * op->Hk;
+  if (s->prep_C != op->C || s->prep_Hk != op->Hk) {
+    std::vector<int2> items;
+    for (int c = 0; c < op->C; ++c)
+      for (int kfe = 0; kfe < op->Hk; ++kfe) items.push_back(make_int2(0, c * op->Hk + kfe));
+    s->prep_items.ensure(items.size() * sizeof(int2));
+    HIPC(hipMemcpyAsync(s->prep_items.p, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));  // the host vector goes out of scope
+    s->prep_C = op->C;
+    s->prep_Hk = op->Hk;
+  }
+  s->prep_part.ensure(size_t(nitems) * kGapTerms * sizeof(double));
+  s->red.ensure(2 * sizeof(double));
   Launch L;
   L.stream = s->stream;
-  L.sl = dsl.as<SliceDev>();
+  L.sl = s->prep_sl.as<SliceDev>();
   L.nsl = 1;
-  L.rows = ditems.as<int2>();
-  L.nrows = (int)items.size();
-  run_compact_data(L, is_dbl(s->dtype) ? DT_C128 : DT_C64, s->f->ptr, part.as<double>());
-  HIPC(hipGetLastError());
-  std::vector<double> hp(items.size() * kGapTerms);
-  HIPC(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  HIPC(hipStreamSynchronize(s->stream));
-  double f0 = 0;
-  for (size_t i = 0; i < items.size(); ++i) f0 += hp[i * kGapTerms];
-  s->F0 = f0;
-  // ||A^* f|| (radius of the partial-gap ball)
+  L.rows = s->prep_items.as<int2>();
+  L.nrows = nitems;
+  const int dt = is_dbl(s->dtype) ? DT_C128 : DT_C64;
+  run_compact_data(L, dt, s->f->ptr, s->prep_part.as<double>());
+  run_sum_strided(s->stream, s->prep_part.as<double>(), nitems, kGapTerms, s->red.as<double>());
+  // ||A^* f|| (radius of the partial-gap ball); A^* f is kept for the cold start
   const size_t n = size_t(op->W) * op->H;
-  DevBuf atf(n * ce);
-  op->apply(false, s->f->ptr, atf.p, s->stream);
-  std::vector<double> h(n * 2);
-  if (is_dbl(s->dtype)) {
-    HIPC(hipMemcpyAsync(h.data(), atf.p, n * ce, hipMemcpyDeviceToHost, s->stream));
-    HIPC(hipStreamSynchronize(s->stream));
-  } else {
-    std::vector<float> hf(n * 2);
-    HIPC(hipMemcpyAsync(hf.data(), atf.p, n * ce, hipMemcpyDeviceToHost, s->stream));
-    HIPC(hipStreamSynchronize(s->stream));
-    for (size_t i = 0; i < hf.size(); ++i) h[i] = hf[i];
-  }
-  double ss = 0;
-  for (double x : h) ss += x * x;
-  s->atf_norm = std::sqrt(ss);
+  s->atf.ensure(n * ce);
+  op->apply(false, s->f->ptr, s->atf.p, s->stream);
+  run_sumsq(s->stream, dt, s->atf.p, int64_t(n), s->red.as<double>() + 1);
+  HIPC(hipGetLastError());
+  double h[2];
+  HIPC(hipMemcpyAsync(h, s->red.p, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+  HIPC(hipStreamSynchronize(s->stream));
+  s->F0 = h[0];
+  s->atf_norm = std::sqrt(h[1]);
   s->rupd = 0;
   s->data_dirty = false;
 }
@@ -1179,10 +1183,9 @@ void prepare_solve(pdt_solver* s, int verbose) {
       // Initialization with A^T(f) (tgvmrioptimizer.cu:112-118)
       if (verbose) std::printf("Initialization with A^T(f)\n");
       const size_t n = size_t(s->op->W) * s->op->H;
-      DevBuf u0(n * s->pe());
-      s->op->apply(false, s->f->ptr, u0.p, s->stream);
-      HIPC(hipStreamSynchronize(s->stream));
-      set_input0(s, u0.p, nullptr, s->op->W, s->op->H);
+      s->atf.ensure(n * s->pe());
+      s->op->apply(false, s->f->ptr, s->atf.p, s->stream);
+      set_input0(s, s->atf.p, nullptr, s->op->W, s->op->H);  // stream-ordered after the apply
     } else {
       const int64_t want[2] = {s->op->W, s->op->H};
       size_check(s->u->size, 2, want, 2);
@@ -1549,7 +1552,11 @@ int pdt_solver_set_noisy_data(pdt_solver* s, const pdt_array* f) {
   return guard([&] {
     if (!s || !f) fail(PDT_ERR_ARG, "null argument");
     if (f->dtype != s->dtype) fail(PDT_ERR_ARG, "data dtype does not match the solver");
-    s->f.reset(new_array(f->dtype, f->ndim, f->size));
+    // same shape as the held copy: overwrite it (no reallocation, and the
+    // address the cached engine holds stays valid)
+    bool same = s->f && s->f->dtype == f->dtype && s->f->ndim == f->ndim;
+    for (int k = 0; same && k < f->ndim; ++k) same = s->f->size[k] == f->size[k];
+    if (!same) s->f.reset(new_array(f->dtype, f->ndim, f->size));
     if (f->bytes()) HIPC(hipMemcpy(s->f->ptr, f->ptr, f->bytes(), hipMemcpyDeviceToDevice));
     s->data_dirty = true;  // r = 0 (tgvmrioptimizer.cu:88-90)
   });
@@ -1800,8 +1807,12 @@ struct pdt_cnn {
   int64_t cap = 0;             // pixels the activation buffers hold
   DevBuf xb, y, dza, dzb, part, lpart;
   std::vector<DevBuf> act;     // hidden activations (bf16), nl - 1 of them
+  DevBuf ws_x, ws_u, ws_v, ws_ss;  // warm-start input, (u0, v0) and sum of |A^*f|^2
   hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;  // solver stream <-> network stream ordering
   ~pdt_cnn() {
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -2029,22 +2040,27 @@ int pdt_solver_warm_start(pdt_solver* s, pdt_cnn* net) {
     prepare_solve(s, 0);
     const int W = s->op->W, H = s->op->H;
     const int64_t n = int64_t(W) * H;
-    DevBuf u0(size_t(n) * 8), x(size_t(n) * 2 * 4), u(size_t(n) * 8), v(size_t(n) * 16), part(cnn_sumsq_blocks() * 8);
-    s->op->apply(false, s->f->ptr, u0.p, s->stream);
-    cnn_ws_sumsq(s->stream, u0.p, n, part.as<double>());
-    std::vector<double> hp(cnn_sumsq_blocks());
-    HIPC(hipMemcpyAsync(hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost, s->stream));
-    HIPC(hipStreamSynchronize(s->stream));
-    double ss = 0;
-    for (double q : hp) ss += q;
-    const double scale = ss > 0 ? std::sqrt(ss / double(n)) : 1.0;
-    cnn_ws_pack(s->stream, u0.p, n, float(1.0 / scale), x.as<float>());
-    HIPC(hipStreamSynchronize(s->stream));
-    net->forward(x.as<float>(), 1, H, W);
-    cnn_ws_unpack(net->stream, x.as<float>(), net->y.as<float>(), n, float(scale), u.p, v.p);
+    // A^*f and its sum of squares on the solver stream; the scale RMS(A^*f)
+    // is taken on the device by the pack and unpack kernels (no host trip)
+    s->atf.ensure(size_t(n) * 8);
+    net->ws_x.ensure(size_t(n) * 2 * 4);
+    net->ws_u.ensure(size_t(n) * 8);
+    net->ws_v.ensure(size_t(n) * 16);
+    net->ws_ss.ensure(sizeof(double));
+    if (!net->ev_in) HIPC(hipEventCreateWithFlags(&net->ev_in, hipEventDisableTiming));
+    if (!net->ev_out) HIPC(hipEventCreateWithFlags(&net->ev_out, hipEventDisableTiming));
+    s->op->apply(false, s->f->ptr, s->atf.p, s->stream);
+    run_sumsq(s->stream, DT_C64, s->atf.p, n, net->ws_ss.as<double>());
+    cnn_ws_pack(s->stream, s->atf.p, n, net->ws_ss.as<double>(), net->ws_x.as<float>());
+    HIPC(hipEventRecord(net->ev_in, s->stream));
+    HIPC(hipStreamWaitEvent(net->stream, net->ev_in, 0));
+    net->forward(net->ws_x.as<float>(), 1, H, W);
+    cnn_ws_unpack(net->stream, net->ws_x.as<float>(), net->y.as<float>(), n, net->ws_ss.as<double>(), net->ws_u.p,
+                  net->ws_v.p);
     HIPC(hipGetLastError());
-    HIPC(hipStreamSynchronize(net->stream));
-    set_input0(s, u.p, v.p, W, H);
+    HIPC(hipEventRecord(net->ev_out, net->stream));
+    HIPC(hipStreamWaitEvent(s->stream, net->ev_out, 0));
+    set_input0(s, net->ws_u.p, net->ws_v.p, W, H);  // on the solver stream; synchronises
   });
 }
 

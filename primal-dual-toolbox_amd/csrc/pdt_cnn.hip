@@ -293,32 +293,24 @@ __global__ __launch_bounds__(256) void adam_kernel(float* w, const float* g, flo
 
 // Warm-start input: x[p] = (re, im) of u0[p] / scale (complex64 planar ->
 // NHWC fp32, 2 channels), and per-block partial sums of |u0|^2.
-__global__ __launch_bounds__(256) void ws_pack(const float2* u0, int64_t n, float inv_scale, float* x) {
+// Scale of the warm-start network input: the RMS of A^*f from its sum of
+// squares ss (n pixels), 1 for an all-zero image.
+__device__ __forceinline__ double ws_scale(const double* ss, int64_t n) {
+  return ss[0] > 0 ? sqrt(ss[0] / double(n)) : 1.0;
+}
+__global__ __launch_bounds__(256) void ws_pack(const float2* u0, int64_t n, const double* ss, float* x) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const float inv_scale = float(1.0 / ws_scale(ss, n));
   const float2 z = u0[i];
   x[2 * i] = z.x * inv_scale;
   x[2 * i + 1] = z.y * inv_scale;
 }
-__global__ __launch_bounds__(256) void ws_sumsq(const float2* u0, int64_t n, double* part) {
-  __shared__ double red[256];
-  double s = 0;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
-    s += double(u0[i].x) * u0[i].x + double(u0[i].y) * u0[i].y;
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-// Network output y[p] (6 channels) -> u = scale (x + y[0..1]) and the two
-// planes of v: v_x = scale (y2 + i y3), v_y = scale (y4 + i y5).
-__global__ __launch_bounds__(256) void ws_unpack(const float* x, const float* y, int64_t n, float scale, float2* u,
-                                                 float2* v) {
+__global__ __launch_bounds__(256) void ws_unpack(const float* x, const float* y, int64_t n, const double* ss,
+                                                 float2* u, float2* v) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const float scale = float(ws_scale(ss, n));
   const float* yi = y + 6 * i;
   u[i] = make_float2(scale * (x[2 * i] + yi[0]), scale * (x[2 * i + 1] + yi[1]));
   v[i] = make_float2(scale * yi[2], scale * yi[3]);
@@ -413,20 +405,13 @@ void cnn_adam(hipStream_t st, float* w, const float* g, float* m, float* v, int6
   hipLaunchKernelGGL(adam_kernel, dim3(int((n + 255) / 256)), dim3(256), 0, st, w, g, m, v, n, lr, b1, b2, eps, c1, c2);
 }
 
-constexpr int kSumBlocks = 256;
-int cnn_sumsq_blocks() { return kSumBlocks; }
-
-void cnn_ws_sumsq(hipStream_t st, const void* u0, int64_t n, double* part) {
-  hipLaunchKernelGGL(ws_sumsq, dim3(kSumBlocks), dim3(256), 0, st, static_cast<const float2*>(u0), n, part);
+void cnn_ws_pack(hipStream_t st, const void* u0, int64_t n, const double* ss, float* x) {
+  hipLaunchKernelGGL(ws_pack, dim3(int((n + 255) / 256)), dim3(256), 0, st, static_cast<const float2*>(u0), n, ss,
+                     x);
 }
 
-void cnn_ws_pack(hipStream_t st, const void* u0, int64_t n, float inv_scale, float* x) {
-  hipLaunchKernelGGL(ws_pack, dim3(int((n + 255) / 256)), dim3(256), 0, st, static_cast<const float2*>(u0), n,
-                     inv_scale, x);
-}
-
-void cnn_ws_unpack(hipStream_t st, const float* x, const float* y, int64_t n, float scale, void* u, void* v) {
-  hipLaunchKernelGGL(ws_unpack, dim3(int((n + 255) / 256)), dim3(256), 0, st, x, y, n, scale,
+void cnn_ws_unpack(hipStream_t st, const float* x, const float* y, int64_t n, const double* ss, void* u, void* v) {
+  hipLaunchKernelGGL(ws_unpack, dim3(int((n + 255) / 256)), dim3(256), 0, st, x, y, n, ss,
                      static_cast<float2*>(u), static_cast<float2*>(v));
 }
 

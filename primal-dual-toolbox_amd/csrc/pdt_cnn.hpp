@@ -34,10 +34,9 @@ void cnn_mse(hipStream_t st, const float* y, const float* t, int64_t m, void* dz
 // One Adam update; c1 = 1 - b1^t, c2 = 1 - b2^t.
 void cnn_adam(hipStream_t st, float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
               float eps, float c1, float c2);
-// Warm start from a complex64 image u0 (n pixels).
-int cnn_sumsq_blocks();
-void cnn_ws_sumsq(hipStream_t st, const void* u0, int64_t n, double* part);
-void cnn_ws_pack(hipStream_t st, const void* u0, int64_t n, float inv_scale, float* x);
-void cnn_ws_unpack(hipStream_t st, const float* x, const float* y, int64_t n, float scale, void* u, void* v);
+// Warm start from a complex64 image u0 (n pixels); ss points to the sum of
+// |u0|^2 on the device, from which the input scale RMS(u0) is taken.
+void cnn_ws_pack(hipStream_t st, const void* u0, int64_t n, const double* ss, float* x);
+void cnn_ws_unpack(hipStream_t st, const float* x, const float* y, int64_t n, const double* ss, void* u, void* v);
 
 }  // namespace pdt

@@ -1858,6 +1858,39 @@ int run_compact_data(const Launch& L, int dtype, const void* f_full, double* f0_
   return L.nrows;
 }
 
+// Single-block reductions for set-up quantities, in a fixed order (each
+// thread sums a strided subsequence, then a tree over the block): out[0] =
+// sum_i a[i * stride] for i < n, and out[0] = sum_i |z_i|^2 over n pixels.
+__device__ __forceinline__ void tree_sum_1024(double v, double* out) {
+  __shared__ double red[1024];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+__global__ __launch_bounds__(1024) void k_sum_strided(const double* a, int n, int stride, double* out) {
+  double v = 0;
+  for (int i = threadIdx.x; i < n; i += 1024) v += a[size_t(i) * stride];
+  tree_sum_1024(v, out);
+}
+template <class R>
+__global__ __launch_bounds__(1024) void k_sumsq(const Cx<R>* z, int64_t n, double* out) {
+  double v = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) v += dabs2(z[i]);
+  tree_sum_1024(v, out);
+}
+
+void run_sum_strided(hipStream_t st, const double* a, int n, int stride, double* out) {
+  PDT_LAUNCH_NT(k_sum_strided, 1, 1024, 0, st, a, n, stride, out);
+}
+void run_sumsq(hipStream_t st, int dtype, const void* z, int64_t n, double* out) {
+  if (dtype == DT_C128) PDT_LAUNCH_NT((k_sumsq<double>), 1, 1024, 0, st, static_cast<const Cx<double>*>(z), n, out);
+  else PDT_LAUNCH_NT((k_sumsq<float>), 1, 1024, 0, st, static_cast<const Cx<float>*>(z), n, out);
+}
+
 void run_zero(hipStream_t st, void* ptr, size_t bytes) {
   const size_t n = bytes / 4;
   if (n == 0) return;

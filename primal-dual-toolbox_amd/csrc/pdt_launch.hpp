@@ -60,6 +60,10 @@ void run_op_adjoint(const Launch& L, int dtype);
 int run_compact_data(const Launch& L, int dtype, const void* f_full, double* f0_part);
 // Fills n elements of the given size (bytes per element 4, 8 or 16) with 0.
 void run_zero(hipStream_t st, void* ptr, size_t bytes);
+// out[0] = sum of a[i * stride], i < n (one block, fixed order).
+void run_sum_strided(hipStream_t st, const double* a, int n, int stride, double* out);
+// out[0] = sum of |z_i|^2 over n complex pixels (DT_C64 or DT_C128).
+void run_sumsq(hipStream_t st, int dtype, const void* z, int64_t n, double* out);
 
 // Allows the MRI kernels the full LDS of a workgroup (call before launches);
 // returns the dynamic LDS bytes a workgroup may use.
