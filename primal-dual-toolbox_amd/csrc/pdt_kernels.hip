@@ -1010,8 +1010,11 @@ __device__ __forceinline__ void copy_to_lds(T* dst, A* src, int n) {
 // each on its own range of pixels.  Then for
 // each coil of the group: S_c * src row -> ifftshift -> PE-FFT -> fftshift
 // -> sampled columns -> T1 [c][y][ks].
+// The gap-evaluation variant holds extra FP64 accumulators; it is asked to
+// fit five waves per SIMD, as many as the plain variant, so that the same
+// tile shapes keep every block resident.
 template <class R, int MODE, bool GAP, int NT, int NF>
-__global__ __launch_bounds__(NT) void k_rows_fwd(const SliceDev* sl, const int2* items, int nitems, int cb,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GAP ? 5 : 1))) void k_rows_fwd(const SliceDev* sl, const int2* items, int nitems, int cb,
                                                  double* part) {
   extern __shared__ char smem_raw[];
   const int li = xcd_item(blockIdx.x, nitems);
@@ -1483,11 +1486,25 @@ __global__ __launch_bounds__(kBlock) void k_gap_reduce(const SliceDev* sl, int k
   double s[kGapTerms] = {0};
   double mx = 0;
   if (mri) {
-    for (int i = d.rg_beg + threadIdx.x; i < d.rg_end; i += kBlock) {
-      const double* a = p0 + size_t(i) * kGapTerms;
-      s[0] += a[0];
-      s[1] += a[1];
-      s[2] += a[2];
+    // four items per step, their loads issued before the additions (which
+    // keep the order of a plain strided loop)
+    for (int i0 = d.rg_beg + threadIdx.x; i0 < d.rg_end; i0 += 4 * kBlock) {
+      double a[4][3];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * kBlock;
+        const bool in = i < d.rg_end;
+        const double* q = p0 + size_t(in ? i : i0) * kGapTerms;
+        a[u][0] = in ? q[0] : 0.0;
+        a[u][1] = in ? q[1] : 0.0;
+        a[u][2] = in ? q[2] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        s[0] += a[u][0];
+        s[1] += a[u][1];
+        s[2] += a[u][2];
+      }
     }
   }
   for (int i = d.row_beg + threadIdx.x; i < d.row_end; i += kBlock) {
@@ -1510,9 +1527,21 @@ __global__ __launch_bounds__(kBlock) void k_gap_reduce(const SliceDev* sl, int k
     }
   }
   if (mri) {
-    for (int i = d.col_beg + threadIdx.x; i < d.col_end; i += kBlock) {
-      s[3] += p1[size_t(i) * kGapTerms + 0];
-      s[4] += p1[size_t(i) * kGapTerms + 1];
+    for (int i0 = d.col_beg + threadIdx.x; i0 < d.col_end; i0 += 4 * kBlock) {
+      double a[4][2];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * kBlock;
+        const bool in = i < d.col_end;
+        const double* q = p1 + size_t(in ? i : i0) * kGapTerms;
+        a[u][0] = in ? q[0] : 0.0;
+        a[u][1] = in ? q[1] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        s[3] += a[u][0];
+        s[4] += a[u][1];
+      }
     }
   }
   __shared__ double tot[kGapTerms];
@@ -1871,15 +1900,34 @@ __device__ __forceinline__ void tree_sum_1024(double v, double* out) {
   }
   if (threadIdx.x == 0) out[0] = red[0];
 }
+// (Eight strided elements per step, loaded before they are added in order.)
 __global__ __launch_bounds__(1024) void k_sum_strided(const double* a, int n, int stride, double* out) {
   double v = 0;
-  for (int i = threadIdx.x; i < n; i += 1024) v += a[size_t(i) * stride];
+  for (int i0 = threadIdx.x; i0 < n; i0 += 8 * 1024) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * 1024;
+      x[u] = i < n ? a[size_t(i) * stride] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v += x[u];
+  }
   tree_sum_1024(v, out);
 }
 template <class R>
 __global__ __launch_bounds__(1024) void k_sumsq(const Cx<R>* z, int64_t n, double* out) {
   double v = 0;
-  for (int64_t i = threadIdx.x; i < n; i += 1024) v += dabs2(z[i]);
+  for (int64_t i0 = threadIdx.x; i0 < n; i0 += 8 * 1024) {
+    Cx<R> x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t i = i0 + u * 1024;
+      x[u] = i < n ? z[i] : pzero<Cx<R>>();
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v += dabs2(x[u]);
+  }
   tree_sum_1024(v, out);
 }
 
