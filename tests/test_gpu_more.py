@@ -328,6 +328,49 @@ def test_resolve_continues_and_input0_warm_start(oracle):
     assert rel_l2(c.getResult(), ref) < 1e-3
 
 
+def test_new_data_same_shape_equals_fresh_solver():
+    """setNoisyData with new data of the same shape overwrites the held copy
+    and the cached engine (same buffers) must pick the new data up: the
+    result equals that of a fresh solver, for denoising (data read by the
+    kernels directly) and for MRI (data compacted per solve)."""
+    pdt = _pdt()
+    from primaldualtoolbox import synthetic as syn
+    rng = np.random.default_rng(12)
+    f1, f2 = (rng.standard_normal((2, 24, 20)) + 1j * rng.standard_normal((2, 24, 20))).astype(np.complex64)
+
+    def tgv():
+        o = pdt.denoising.TgvOptimizer_2c()
+        p = o.getParameters()
+        p.Lambda, p.alpha0, p.alpha1, p.max_iter = 10, 2, 1, 49
+        return o
+
+    a = tgv()
+    a.setNoisyData(f1)
+    a.solve()
+    a.setNoisyData(f2)
+    a.setInput0(f2)
+    a.solve()
+    b = tgv()
+    b.setNoisyData(f2)
+    b.solve()
+    assert np.array_equal(a.getResult(), b.getResult())
+    c1 = syn.mri_case(seed=21, n=32, coils=3, center=4)
+    c2 = syn.mri_case(seed=22, n=32, coils=3, center=4)
+    c2["mask"], c2["sens"] = c1["mask"], c1["sens"]  # same operator, new data
+    a1 = 1e-3 * float(np.linalg.norm(c1["kspace"]))
+    _, m = _mri_solver(pdt, c1, a1=a1, max_iter=29)
+    m.solve()
+    op = pdt.mri.MriCartesianRemoveROOSOperator()
+    op.setCoilSens(c1["sens"])
+    op.setMask(c1["mask"])
+    m.setNoisyData(c2["kspace"])
+    m.setInput0(op.adjoint(c2["kspace"]))
+    m.solve()
+    _, n = _mri_solver(pdt, {**c1, "kspace": c2["kspace"]}, a1=a1, max_iter=29)
+    n.solve()
+    assert np.array_equal(m.getResult(), n.getResult())
+
+
 def test_operator_constant_and_size_errors():
     """operatorbase.h:242-256,276-287,493-502; mricartesianoperator.cu:310-318."""
     pdt = _pdt()
