@@ -1305,7 +1305,22 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   // VGPRs (two blocks per CU), so it holds fewer
   constexpr int PS = NT >= 1024 ? 6 : kPrefetch, PT = NT >= 1024 ? 2 : 4;
   C pS[PS];
-  const bool pf = NF > 0 && Sg && ncb * W <= PS * NT;
+  // Coil sum in registers (one block holds the whole row, u step fused):
+  // thread t = gi W + x takes pixel x of the coils [gi ch, (gi + 1) ch) of
+  // the row, multiplies and sums them after the FFT, and the ng partial sums
+  // of a pixel are added in group order.  Otherwise every thread takes
+  // elements e = cl W + x in turn and one thread per pixel sums all coils.
+  const int ngs = min(NT / W, ncb), ch = (ncb + ngs - 1) / max(ngs, 1);
+  const bool gsum = NF > 0 && Sg && MODE != M_OP && d.ngr_a == 1 && 2 * W <= NT && ch <= PS;
+  const int gi = threadIdx.x / W, gx = threadIdx.x - gi * W;
+  if (gsum) {
+#pragma unroll
+    for (int k = 0; k < PS; ++k) {
+      const int cl = gi * ch + k;
+      if (gi < ngs && k < ch && cl < ncb) pS[k] = Sg[cl * G + gx];
+    }
+  }
+  const bool pf = !gsum && NF > 0 && Sg && ncb * W <= PS * NT;
   if (pf) {
 #pragma unroll
     for (int k = 0; k < PS; ++k) {
@@ -1379,7 +1394,17 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   __syncthreads();
   C* res = fft_n<R, NT, true, NF, LP>(L.a, L.b, ncb, d.planW, L.tw);
   C* other = (res == L.a) ? L.b : L.a;
-  if (pf) {
+  if (gsum) {
+    if (gi < ngs) {
+      C acc = pzero<C>();
+#pragma unroll
+      for (int k = 0; k < PS; ++k) {
+        const int cl = gi * ch + k;
+        if (k < ch && cl < ncb) acc = acc + mulconj(res[lp<NF, LP>(cl * W + fshift_src(gx, W))], pS[k]);
+      }
+      other[gi * W + gx] = acc;
+    }
+  } else if (pf) {
 #pragma unroll
     for (int k = 0; k < PS; ++k) {
       const int e = threadIdx.x + k * NT;
@@ -1408,8 +1433,14 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
   if (early) {
     const int x = threadIdx.x;
     if (x < W) {
-      C w = other[lp<NF, LP>(x)];
-      for (int cl = 1; cl < ncb; ++cl) w = w + other[lp<NF, LP>(cl * W + x)];
+      C w;
+      if (gsum) {
+        w = other[x];
+        for (int q = 1; q < ngs; ++q) w = w + other[q * W + x];
+      } else {
+        w = other[lp<NF, LP>(x)];
+        for (int cl = 1; cl < ncb; ++cl) w = w + other[lp<NF, LP>(cl * W + x)];
+      }
       const size_t i = size_t(y) * W + x;
       const C un = e_uo + tau * (e_div - w);
       gptr<C>(d.u)[i] = un;
