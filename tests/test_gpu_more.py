@@ -86,19 +86,30 @@ def _tgv512_ref(oracle, noisy):
     return _CFG2["u"]
 
 
-def test_mri_320_full_size_parity_fixed_iterations(oracle):
-    """320x320, 15 coils, ROOS 640: 20 iterations vs the FP64 oracle."""
+@pytest.mark.parametrize("tiles", [None, (5, 1, 15), (4, 2, 5)])
+def test_mri_320_full_size_parity_fixed_iterations(oracle, tiles):
+    """320x320, 15 coils, ROOS 640: 20 iterations vs the FP64 oracle, with
+    the default tiles and with forced ones: (5, 1, 15) is the autotuned shape
+    of this workload (all 15 coils of a row in one 1024-thread adjoint block,
+    u and v steps fused, coil products summed by three thread groups),
+    (4, 2, 5) the path with per-group partial adjoints and k_mri_primal."""
     pdt = _pdt()
     from primaldualtoolbox import synthetic as syn
     case = syn.mri_case(seed=0, n=320, coils=15)
     a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
     _, opt = _mri_solver(pdt, case, a1=a1, max_iter=19)
-    opt.solve()
+    if tiles:
+        pdt.mri.set_tiles([opt], cb=tiles[0], cg=tiles[1], cb_a=tiles[2])
+    try:
+        opt.solve()
+    finally:
+        if tiles:
+            pdt.mri.set_tiles([opt])
     oop = oracle.Op(oracle.OP_CARTESIAN_ROOS, case["sens"].astype(np.complex128), case["mask"].astype(np.float64))
     ref, _, _, _ = oracle.solve(oracle.TGV_MRI, case["kspace"].astype(np.complex128), op=oop, alpha1=a1,
                                 alpha0=2 * a1, max_iter=19)
     err = rel_l2(opt.getResult(), ref)
-    print(f"mri 320 20 iterations: rel L2 {err:.3e}", flush=True)
+    print(f"mri 320 20 iterations, tiles {tiles}: rel L2 {err:.3e}", flush=True)
     assert err < 1e-3
 
 
