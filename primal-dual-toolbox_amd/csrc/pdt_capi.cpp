@@ -972,6 +972,14 @@ struct Engine {
         x.adj_threads = x.cb_a * d0.W >= 8 * kRowThreads ? kRowAdjThreadsWide : kRowThreads;
         x.lds_rows_a = lds_rows_bytes(dt, x.cb_a, d0.W, nc, x.adj_threads != kRowAdjThreadsWide);
       }
+      if (mri && sv.size() == 1) {
+        auto identity = [](const std::vector<int2>& v) {
+          for (size_t i = 0; i < v.size(); ++i)
+            if (v[i].x != 0 || v[i].y != int(i)) return false;
+          return true;
+        };
+        x.dense = identity(rgs) && identity(rgs_a) && identity(cols);
+      }
       listed_items += r.nrow + r.nrg + r.nrga + r.ncol;
       G.push_back(x);
     }

@@ -1018,7 +1018,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GAP ? 5 : 1)
                                                  double* part) {
   extern __shared__ char smem_raw[];
   const int li = xcd_item(blockIdx.x, nitems);
-  const int2 it = items[li];
+  const int2 it = items ? items[li] : make_int2(0, li);  // null: the dense single-slice list
   const SliceDev& d = sl[it.x];
   using C = Cx<R>;
   const int W = NF > 0 ? NF : d.W, H = d.H, nc = d.ncols;
@@ -1114,7 +1114,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
                                              double* part) {
   extern __shared__ char smem_raw[];
   const int li = xcd_item(blockIdx.x, nitems);
-  const int2 it = items[li];
+  const int2 it = items ? items[li] : make_int2(0, li);  // null: the dense single-slice list
   const SliceDev& d = sl[it.x];
   using C = Cx<R>;
   const int W = d.W, H = d.H, Hk = NF > 0 ? NF : d.Hk, y0 = d.y0, nc = d.ncols;
@@ -1280,7 +1280,7 @@ __global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2*
                                                  double* part_rows) {
   extern __shared__ char smem_raw[];
   const int li = xcd_item(blockIdx.x, nitems);
-  const int2 it = items[li];
+  const int2 it = items ? items[li] : make_int2(0, li);  // null: the dense single-slice list
   const SliceDev& d = sl[it.x];
   using C = Cx<R>;
   const int W = NF > 0 ? NF : d.W, H = d.H, nc = d.ncols;
@@ -1746,11 +1746,11 @@ template <class R, int MODE, int NF>
 static void launch_rows_adj(const Launch& L, bool gap) {
   constexpr int NR = kRowThreads, NW = kRowAdjThreadsWide;
   if (L.adj_threads == NW) {
-    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NW, NF>), L.nrg_a, NW, L.lds_rows_a, L.stream, L.sl, L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
-    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NW, NF>), L.nrg_a, NW, L.lds_rows_a, L.stream, L.sl, L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
+    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NW, NF>), L.nrg_a, NW, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
+    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NW, NF>), L.nrg_a, NW, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
   } else {
-    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NR, NF>), L.nrg_a, NR, L.lds_rows_a, L.stream, L.sl, L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
-    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NR, NF>), L.nrg_a, NR, L.lds_rows_a, L.stream, L.sl, L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
+    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NR, NF>), L.nrg_a, NR, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
+    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NR, NF>), L.nrg_a, NR, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
   }
 }
 
@@ -1762,15 +1762,15 @@ static void mri_stage_m(const Launch& L, bool gap, int stage) {
   if (stage == 0) {
     with_fft_len<R>(L.fft_w, [&](auto nf) {
       constexpr int NF = decltype(nf)::value;
-      if (gap) PDT_LAUNCH_NT((k_rows_fwd<R, MODE, true, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0]);
-      else PDT_LAUNCH_NT((k_rows_fwd<R, MODE, false, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0]);
+      if (gap) PDT_LAUNCH_NT((k_rows_fwd<R, MODE, true, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.dense ? nullptr : L.rowgroups, L.nrg, L.cb, L.part[0]);
+      else PDT_LAUNCH_NT((k_rows_fwd<R, MODE, false, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.dense ? nullptr : L.rowgroups, L.nrg, L.cb, L.part[0]);
     });
   } else if (stage == 1) {
     if (L.ncolitems == 0) return;
     with_fft_len<R>(L.fft_h, [&](auto nf) {
       constexpr int NF = decltype(nf)::value;
-      if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
-      else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+      if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.dense ? nullptr : L.cols, L.ncolitems, L.cg, L.part[1]);
+      else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.dense ? nullptr : L.cols, L.ncolitems, L.cg, L.part[1]);
     });
   } else if (stage == 2) {
     with_fft_len<R>(L.fft_w, [&](auto nf) {
