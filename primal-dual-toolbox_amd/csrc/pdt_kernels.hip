@@ -9,7 +9,7 @@
 //    kernels per iteration.
 //      rows_fwd  : dual p (and q) update of image row y, then for every coil
 //                  S_c * u_bar row, ifftshift, PE-FFT in LDS, fftshift, keep
-//                  the sampled PE columns                  -> T1 [c][y][ks]
+//                  the sampled PE columns                  -> T1 [c][ks][y]
 //      cols      : per coil and group of sampled columns: FE-FFT of the
 //                  zero-padded (read-out oversampled) column, mask, data prox
 //                  of r, mask, inverse FE-FFT, crop to the image window
@@ -1009,7 +1009,7 @@ __device__ __forceinline__ void copy_to_lds(T* dst, A* src, int n) {
 // y.  The groups of row y also perform the dual step of row y (TV or TGV),
 // each on its own range of pixels.  Then for
 // each coil of the group: S_c * src row -> ifftshift -> PE-FFT -> fftshift
-// -> sampled columns -> T1 [c][y][ks].
+// -> sampled columns -> T1 [c][ks][y].
 // The gap-evaluation variant holds extra FP64 accumulators; it is asked to
 // fit five waves per SIMD, as many as the plain variant, so that the same
 // tile shapes keep every block resident.
@@ -1096,11 +1096,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GAP ? 5 : 1)
   }
   __syncthreads();
   C* res = fft_n<R, NT, false, NF>(L.a, L.b, ncb, d.planW, L.tw);
-  const auto T1 = gptr<C>(d.T1) + (size_t(c0) * H + y) * nc;  // row y of coil c0
-  const int cstride = H * nc;
+  // T1 is column-major per coil, [c][ks][y], so that the column pass reads
+  // its columns contiguously; here each row writes one element per column
+  const auto T1 = gptr<C>(d.T1) + size_t(c0) * nc * H + y;  // coil c0, column 0, row y
   for (int e = threadIdx.x; e < ncb * nc; e += NT) {
     const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
-    T1[cl * cstride + ks] = res[lp<NF>(cl * W + fshift_src(L.cols[ks], W))];
+    T1[size_t(cl * nc + ks) * H] = res[lp<NF>(cl * W + fshift_src(L.cols[ks], W))];
   }
   if (GAP) block_sum<3, NT>(reinterpret_cast<double(&)[3]>(t), part + size_t(li) * kGapTerms);
 }
@@ -1157,7 +1158,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   }
   if (!slice_active(d)) return;  // after the prefetch is issued, before any store
   if (CMODE != CM_ADJ) {
-    const auto T1 = gptr<const C>(d.T1) + size_t(c) * H * nc + ks0;  // coil c, column ks0
+    const auto T1 = gptr<const C>(d.T1) + (size_t(c) * nc + ks0) * H;  // coil c, column ks0 (contiguous in y)
     // the gather of the zero-padded, ifftshifted columns: into registers with
     // the prefetch (one latency for both), else straight into LDS
     C pf_t[KP];
@@ -1168,7 +1169,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
         if (e < ncg * Hk) {
           const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
           const int yy = ishift_src(j, Hk) - y0;
-          pf_t[k] = (yy >= 0 && yy < H) ? C(T1[yy * nc + cl]) : pzero<C>();
+          pf_t[k] = (yy >= 0 && yy < H) ? C(T1[cl * H + yy]) : pzero<C>();
         }
       }
 #pragma unroll
@@ -1184,7 +1185,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
         const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
         const int fe = ishift_src(j, Hk);
         const int yy = fe - y0;
-        bufA[lp<NF>(cl * Hk + j)] = (yy >= 0 && yy < H) ? C(T1[yy * nc + cl]) : pzero<C>();
+        bufA[lp<NF>(cl * Hk + j)] = (yy >= 0 && yy < H) ? C(T1[cl * H + yy]) : pzero<C>();
       }
     }
     if (NF > 0) s_tw.store(tw, Hk);
