@@ -691,6 +691,11 @@ struct Engine {
   DevBuf d_slices, d_rows, d_rg, d_rg_a, d_cols, d_part, d_ctl, d_stats;
   int* h_ctl = nullptr;       // pinned mirrors
   double* h_stats = nullptr;
+  // single-slice pipelining: two pinned copies of the control words and
+  // statistics (one per chunk in flight) and their completion events
+  int* h_ctl_q[2] = {nullptr, nullptr};
+  double* h_stats_q[2] = {nullptr, nullptr};
+  hipEvent_t ev_q[2] = {nullptr, nullptr};
   int nrows = 0, nrg = 0, nrg_a = 0, ncolitems = 0;
   Launch L;                             // every slice: gap reduction, full-size buffers
   std::vector<Launch> G;                // one launch per group of slices with equal FFT lengths
@@ -708,6 +713,11 @@ struct Engine {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (h_ctl) (void)hipHostFree(h_ctl);
     if (h_stats) (void)hipHostFree(h_stats);
+    for (int q = 0; q < 2; ++q) {
+      if (h_ctl_q[q]) (void)hipHostFree(h_ctl_q[q]);
+      if (h_stats_q[q]) (void)hipHostFree(h_stats_q[q]);
+      if (ev_q[q]) (void)hipEventDestroy(ev_q[q]);
+    }
   }
 
   void build() {
@@ -972,13 +982,13 @@ struct Engine {
         x.adj_threads = x.cb_a * d0.W >= 8 * kRowThreads ? kRowAdjThreadsWide : kRowThreads;
         x.lds_rows_a = lds_rows_bytes(dt, x.cb_a, d0.W, nc, x.adj_threads != kRowAdjThreadsWide);
       }
-      if (mri && sv.size() == 1) {
-        auto identity = [](const std::vector<int2>& v) {
+      if (sv.size() == 1) {
+        auto identity = [](const std::vector<int2>& v, int step) {
           for (size_t i = 0; i < v.size(); ++i)
-            if (v[i].x != 0 || v[i].y != int(i)) return false;
+            if (v[i].x != 0 || v[i].y != int(i) * step) return false;
           return true;
         };
-        x.dense = identity(rgs) && identity(rgs_a) && identity(cols);
+        x.dense = mri ? identity(rgs, 1) && identity(rgs_a, 1) && identity(cols, 1) : identity(rows, L.rpi);
       }
       listed_items += r.nrow + r.nrg + r.nrga + r.ncol;
       G.push_back(x);
@@ -1072,9 +1082,14 @@ struct Engine {
       iteration(g);
       if (g) run_gap_reduce(L, kind, dt);
     }
+  }
+
+  // Control words and statistics of every slice to pinned host memory
+  // (stream-ordered after the chunk enqueued before).
+  void copy_ctl(int* hc, double* hst) {
     const int ns = (int)sv.size();
-    HIPC(hipMemcpyAsync(h_ctl, d_ctl.p, size_t(ns) * CTL_WORDS * sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPC(hipMemcpyAsync(h_stats, d_stats.p, size_t(ns) * ST_WORDS * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPC(hipMemcpyAsync(hc, d_ctl.p, size_t(ns) * CTL_WORDS * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPC(hipMemcpyAsync(hst, d_stats.p, size_t(ns) * ST_WORDS * sizeof(double), hipMemcpyDeviceToHost, stream));
   }
 
   void launch_chunk(int len, bool gap, bool use_graph) {
@@ -1109,48 +1124,93 @@ struct Engine {
       limit[i] = sv[i]->prm.max_iter + 1;  // the reference runs k = 0..max_iter
       total = std::max(total, limit[i]);
     }
-    const unsigned chunk = gap ? gap_every : std::min<unsigned>(total, 50);
+    // Graphs hold at most kChunk iterations (a long graph's launch latency is
+    // not hidden); the gap is evaluated at the multiples of gap_every.
+    constexpr unsigned kChunk = 50;
+    const unsigned chunk = std::min<unsigned>(gap ? gap_every : total, kChunk);
     const bool use_graph = sv[0]->prm.use_graph != 0;
     reset_ctl();
     hipEvent_t e0, e1;
     HIPC(hipEventCreate(&e0));
     HIPC(hipEventCreate(&e1));
     HIPC(hipEventRecord(e0, stream));
-    unsigned k = 0;
-    while (k < total) {
-      // chunk boundary: next multiple of the chunk, or the next slice limit
+    auto boundary = [&](unsigned k) {
       unsigned b = std::min(total, (k / chunk + 1) * chunk);
+      if (gap) b = std::min(b, (k / gap_every + 1) * gap_every);
       for (int i = 0; i < ns; ++i)
         if (limit[i] > k) b = std::min(b, limit[i]);
-      const int len = int(b - k);
-      const bool g = gap && (b % gap_every) == 0;
-      launch_chunk(len, g, use_graph);
-      HIPC(hipStreamSynchronize(stream));
-      k = b;
-      // a slice stops at its own iteration limit (host-driven, so that no
-      // kernel ever reads a control word that another block of the same
-      // launch writes)
-      bool changed = false, any = false;
-      for (int i = 0; i < ns; ++i) {
-        int* c = h_ctl + size_t(i) * CTL_WORDS;
-        if (c[CTL_ACTIVE] && limit[i] <= k) {
-          c[CTL_ACTIVE] = 0;
-          changed = true;
-        }
-        any = any || c[CTL_ACTIVE];
+      return b;
+    };
+    auto report = [&]() {
+      if (!verbose) return;
+      for (int i = 0; i < std::min(ns, 4); ++i) {
+        const double* st = h_stats + size_t(i) * ST_WORDS;
+        std::printf("[pdt] slice %d iter=%d P=%.9g D=%.9g rel_gap=%.3e\n", i, h_ctl[i * CTL_WORDS + CTL_ITER],
+                    st[ST_P], st[ST_D], st[ST_REL]);
       }
-      if (verbose) {
-        for (int i = 0; i < std::min(ns, 4); ++i) {
-          const double* st = h_stats + size_t(i) * ST_WORDS;
-          std::printf("[pdt] slice %d iter=%d P=%.9g D=%.9g rel_gap=%.3e\n", i, h_ctl[i * CTL_WORDS + CTL_ITER],
-                      st[ST_P], st[ST_D], st[ST_REL]);
-        }
-        std::fflush(stdout);
+      std::fflush(stdout);
+    };
+    if (ns == 1) {
+      // One slice: its iteration limit is the end of the loop, so no host
+      // decision is needed before the next chunk.  One chunk stays queued
+      // behind the one whose control words the host is reading; once the
+      // slice has stopped, the queued chunk's kernels return at once.
+      for (int q = 0; q < 2; ++q) {
+        if (!h_ctl_q[q]) HIPC(hipHostMalloc((void**)&h_ctl_q[q], CTL_WORDS * sizeof(int), hipHostMallocDefault));
+        if (!h_stats_q[q]) HIPC(hipHostMalloc((void**)&h_stats_q[q], ST_WORDS * sizeof(double), hipHostMallocDefault));
+        if (!ev_q[q]) HIPC(hipEventCreateWithFlags(&ev_q[q], hipEventDisableTiming));
       }
-      if (!any) break;
-      if (changed)
-        HIPC(hipMemcpyAsync(d_ctl.p, h_ctl, size_t(ns) * CTL_WORDS * sizeof(int), hipMemcpyHostToDevice, stream));
-      compact_lists();
+      int queued[2], nq = 0, head = 0, next = 0;
+      unsigned k = 0;
+      bool live = true;
+      while (true) {
+        while (live && k < total && nq < 2) {
+          const unsigned b = boundary(k);
+          launch_chunk(int(b - k), gap && (b % gap_every) == 0, use_graph);
+          const int q = next;
+          next ^= 1;
+          copy_ctl(h_ctl_q[q], h_stats_q[q]);
+          HIPC(hipEventRecord(ev_q[q], stream));
+          queued[(head + nq) % 2] = q;
+          ++nq;
+          k = b;
+        }
+        if (nq == 0) break;
+        const int q = queued[head];
+        head = (head + 1) % 2;
+        --nq;
+        HIPC(hipEventSynchronize(ev_q[q]));
+        std::memcpy(h_ctl, h_ctl_q[q], CTL_WORDS * sizeof(int));
+        std::memcpy(h_stats, h_stats_q[q], ST_WORDS * sizeof(double));
+        report();
+        if (!h_ctl[CTL_ACTIVE]) live = false;
+      }
+    } else {
+      unsigned k = 0;
+      while (k < total) {
+        const unsigned b = boundary(k);
+        launch_chunk(int(b - k), gap && (b % gap_every) == 0, use_graph);
+        copy_ctl(h_ctl, h_stats);
+        HIPC(hipStreamSynchronize(stream));
+        k = b;
+        // a slice stops at its own iteration limit (host-driven, so that no
+        // kernel ever reads a control word that another block of the same
+        // launch writes)
+        bool changed = false, any = false;
+        for (int i = 0; i < ns; ++i) {
+          int* c = h_ctl + size_t(i) * CTL_WORDS;
+          if (c[CTL_ACTIVE] && limit[i] <= k) {
+            c[CTL_ACTIVE] = 0;
+            changed = true;
+          }
+          any = any || c[CTL_ACTIVE];
+        }
+        report();
+        if (!any) break;
+        if (changed)
+          HIPC(hipMemcpyAsync(d_ctl.p, h_ctl, size_t(ns) * CTL_WORDS * sizeof(int), hipMemcpyHostToDevice, stream));
+        compact_lists();
+      }
     }
     HIPC(hipEventRecord(e1, stream));
     HIPC(hipEventSynchronize(e1));

@@ -765,7 +765,7 @@ __device__ __forceinline__ bool slice_active(const SliceDev& d) { return gptr<co
 template <class P, bool GAP>
 __global__ __launch_bounds__(kBlock) void k_tv_dual(const SliceDev* sl, const int2* items, int rpi,
                                                     double* part) {
-  const int2 it = items[blockIdx.x];
+  const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using R = typename RealOf<P>::T;
@@ -793,7 +793,7 @@ __global__ __launch_bounds__(kBlock) void k_tv_dual(const SliceDev* sl, const in
 template <class P, bool GAP>
 __global__ __launch_bounds__(kBlock) void k_tv_primal(const SliceDev* sl, const int2* items, int rpi,
                                                       double* part) {
-  const int2 it = items[blockIdx.x];
+  const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using R = typename RealOf<P>::T;
@@ -831,7 +831,7 @@ __global__ __launch_bounds__(kBlock) void k_tv_primal(const SliceDev* sl, const 
 // then TGV_primal_v_kernel at every pixel.
 template <class P, class S>
 __global__ __launch_bounds__(kBlock) void k_tgv_primal(const SliceDev* sl, const int2* items, int rpi) {
-  const int2 it = items[blockIdx.x];
+  const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using R = typename RealOf<P>::T;
@@ -863,7 +863,7 @@ __global__ __launch_bounds__(kBlock) void k_tgv_primal(const SliceDev* sl, const
 template <class P, class S, bool GAP>
 __global__ __launch_bounds__(kBlock) void k_tgv_dual(const SliceDev* sl, const int2* items, int rpi,
                                                      double* part) {
-  const int2 it = items[blockIdx.x];
+  const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using R = typename RealOf<P>::T;
@@ -891,7 +891,7 @@ __global__ __launch_bounds__(kBlock) void k_tgv_dual(const SliceDev* sl, const i
 template <class P, class S>
 __global__ __launch_bounds__(kBlock) void k_tgv_restore1(const SliceDev* sl, const int2* items, int rpi,
                                                          double* part) {
-  const int2 it = items[blockIdx.x];
+  const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   const int W = d.W, H = d.H;
@@ -922,7 +922,7 @@ __global__ __launch_bounds__(kBlock) void k_tgv_restore1(const SliceDev* sl, con
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_tgv_restore2(const SliceDev* sl, const int2* items, int rpi,
                                                          double* part) {
-  const int2 it = items[blockIdx.x];
+  const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   const int W = d.W, H = d.H;
@@ -1695,21 +1695,21 @@ static void denoise_stage(const Launch& L, int kind, bool gap, int stage) {
   if (n == 0) return;
   if (kind == K_TV) {
     if (stage == 0) {
-      if (gap) PDT_LAUNCH((k_tv_dual<P, true>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[0]);
-      else PDT_LAUNCH((k_tv_dual<P, false>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[0]);
+      if (gap) PDT_LAUNCH((k_tv_dual<P, true>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[0]);
+      else PDT_LAUNCH((k_tv_dual<P, false>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[0]);
     } else {
-      if (gap) PDT_LAUNCH((k_tv_primal<P, true>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[1]);
-      else PDT_LAUNCH((k_tv_primal<P, false>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[1]);
+      if (gap) PDT_LAUNCH((k_tv_primal<P, true>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[1]);
+      else PDT_LAUNCH((k_tv_primal<P, false>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[1]);
     }
   } else {
     if (stage == 0) {
-      PDT_LAUNCH((k_tgv_primal<P, S>), n, 0, L.stream, L.sl, L.rows, L.rpi);
+      PDT_LAUNCH((k_tgv_primal<P, S>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi);
     } else if (gap) {
-      PDT_LAUNCH((k_tgv_dual<P, S, true>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[0]);
-      PDT_LAUNCH((k_tgv_restore1<P, S>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[2]);
-      PDT_LAUNCH((k_tgv_restore2<P>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[3]);
+      PDT_LAUNCH((k_tgv_dual<P, S, true>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[0]);
+      PDT_LAUNCH((k_tgv_restore1<P, S>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[2]);
+      PDT_LAUNCH((k_tgv_restore2<P>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[3]);
     } else {
-      PDT_LAUNCH((k_tgv_dual<P, S, false>), n, 0, L.stream, L.sl, L.rows, L.rpi, L.part[0]);
+      PDT_LAUNCH((k_tgv_dual<P, S, false>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[0]);
     }
   }
 }

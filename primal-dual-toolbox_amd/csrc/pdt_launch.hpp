@@ -38,8 +38,9 @@ struct Launch {
   int cb_a = 1;
   size_t lds_rows_a = 0;
   int adj_threads = 256;
-  // single-slice engine: every list is (0, i) for i < n, so the row and
-  // column kernels take their item from the block index (no list load)
+  // single-slice engine: every list is (0, i) for i < n ((0, i rpi) for
+  // the denoising rows), so the kernels take their item from the block
+  // index (no list load)
   bool dense = false;
 };
 
