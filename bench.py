@@ -390,7 +390,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(slices shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+              "(slices would shard across GPUs without communication; only the one-GPU share is built)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.calibrate_copy:
