@@ -525,13 +525,22 @@ constexpr bool ct_pad_ok(int n) {
   return true;
 }
 
+// Unpadded FFT buffers (the wide adjoint row pass, whose two buffers fill
+// the LDS share of a workgroup) permute their slots instead: bits 0-2 of an
+// element index are XORed with bits 4-6 and bit 3 with bit 6, a permutation
+// of every aligned block of 16 elements.  The Stockham writes at strides of 8
+// and 64 elements then spread the lanes of a wave over distinct banks, while
+// runs of 16 consecutive elements (the reads) stay on distinct banks.
+__device__ __forceinline__ int swz(int e) { return e ^ ((e >> 4) & 7) ^ (((e >> 6) & 1) << 3); }
+
 // LDS slot of element e of an FFT buffer.  Kernels with a compile-time length
 // (NF > 0) pad one element after every 8: the Stockham stages write with
 // strides of 8 elements (and multiples), which would otherwise put the 64
-// lanes of a wave on a few LDS banks.
+// lanes of a wave on a few LDS banks.  Without padding (LP = false) the
+// slots are permuted by swz.
 template <int NF, bool LP = true>
 __device__ __forceinline__ int lp(int e) {
-  if constexpr (LP && NF > 0 && ct_pad_ok(NF)) return e + (e >> 3);
+  if constexpr (NF > 0 && ct_pad_ok(NF)) return LP ? e + (e >> 3) : swz(e);
   else return e;
 }
 
@@ -552,6 +561,21 @@ __device__ __forceinline__ void butterfly_ct(const Cx<R>* in, Cx<R>* out, int ts
     static_assert(RR % 2 == 1, "even radices are 2, 4, 6, 8 and 10");
     butterfly_odd<R, RR>(in, out, nr, p, tw, ts, inv);
   }
+}
+
+// butterfly_ct on swz-permuted buffers: inputs at logical elements
+// ein + j nr of a, outputs at eout + q p of b.
+template <class R, int RR, bool TW>
+__device__ __forceinline__ void butterfly_ct_swz(const Cx<R>* a, Cx<R>* b, int ein, int eout, int ts,
+                                                 const Cx<R>* tw, bool inv, int nr, int p) {
+  static_assert(RR == 2 || RR == 3 || RR == 4 || RR == 5 || RR == 6 || RR == 8 || RR == 10, "radix");
+  Cx<R> x[RR];
+  x[0] = a[swz(ein)];
+#pragma unroll
+  for (int j = 1; j < RR; ++j) x[j] = TW ? a[swz(ein + j * nr)] * twd(tw, j * ts, inv) : a[swz(ein + j * nr)];
+  Dft<R, RR>::run(x, inv ? R(1) : R(-1));
+#pragma unroll
+  for (int q = 0; q < RR; ++q) b[swz(eout + q * p)] = x[q];
 }
 
 template <class R, int N, int S, int P, int NT, bool INV, bool LP>
@@ -594,8 +618,12 @@ __device__ __forceinline__ Cx<R>* fft_ct_stage(Cx<R>* a, Cx<R>* b, int nseq, con
         const int i = t - seq * nr;
         const int k = i % P;
         const int o = (i - k) * r + k;
-        const int ip = PAD ? i + (i >> 3) : i, op = PAD ? o + (o >> 3) : o;
-        butterfly_ct<R, r, (P > 1)>(a + seq * NP + ip, b + seq * NP + op, k * tws, tw, INV, nrp, pp);
+        if constexpr (!LP && ct_pad_ok(N)) {
+          butterfly_ct_swz<R, r, (P > 1)>(a, b, seq * N + i, seq * N + o, k * tws, tw, INV, nr, P);
+        } else {
+          const int ip = PAD ? i + (i >> 3) : i, op = PAD ? o + (o >> 3) : o;
+          butterfly_ct<R, r, (P > 1)>(a + seq * NP + ip, b + seq * NP + op, k * tws, tw, INV, nrp, pp);
+        }
       }
     }
     __syncthreads();
@@ -1275,9 +1303,11 @@ __device__ __forceinline__ void mri_primal_pixel(const SliceDev& d, int x, int y
 // over the group's coils in order.  With several coil groups (ngr_a > 1) the
 // sum goes to wpart[g][y][x] and k_mri_primal completes the step; with one
 // group the block holds all of (A^* r)(., y) and performs the primal step of
-// row y itself (MODE M_OP: writes A^* r to op_out).
+// row y itself (MODE M_OP: writes A^* r to op_out).  The 1024-thread
+// variant on swz-permuted buffers is held to 64 VGPRs (its slot arithmetic
+// would take more), so that two workgroups share a CU.
 template <class R, int MODE, bool GAP, int NT, int NF>
-__global__ __launch_bounds__(NT) void k_rows_adj(const SliceDev* sl, const int2* items, int nitems, int cb,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1024 && NF > 0 && ct_pad_ok(NF) ? 8 : 1))) void k_rows_adj(const SliceDev* sl, const int2* items, int nitems, int cb,
                                                  double* part_rows) {
   extern __shared__ char smem_raw[];
   const int li = xcd_item(blockIdx.x, nitems);
