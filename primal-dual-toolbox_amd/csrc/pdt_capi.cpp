@@ -1154,7 +1154,17 @@ struct Engine {
       // One slice: its iteration limit is the end of the loop, so no host
       // decision is needed before the next chunk.  One chunk stays queued
       // behind the one whose control words the host is reading; once the
-      // slice has stopped, the queued chunk's kernels return at once.
+      // slice has stopped, the queued chunk's kernels return at once.  When
+      // the last two gap values extrapolate (geometrically) to convergence
+      // at the chunk in flight, nothing is queued behind it: a chunk of
+      // returning launches costs more than the host's wait for the verdict.
+      const double tol = gap ? sv[0]->prm.gap_tol : 0.0;
+      double last_rel = 0, prev_rel = 0;
+      int seen_ngap = 0;
+      auto converging = [&]() {
+        if (tol <= 0 || seen_ngap < 2 || !(prev_rel > 0)) return false;
+        return last_rel * std::min(1.0, last_rel / prev_rel) <= 1.02 * tol;
+      };
       for (int q = 0; q < 2; ++q) {
         if (!h_ctl_q[q]) HIPC(hipHostMalloc((void**)&h_ctl_q[q], CTL_WORDS * sizeof(int), hipHostMallocDefault));
         if (!h_stats_q[q]) HIPC(hipHostMalloc((void**)&h_stats_q[q], ST_WORDS * sizeof(double), hipHostMallocDefault));
@@ -1164,7 +1174,7 @@ struct Engine {
       unsigned k = 0;
       bool live = true;
       while (true) {
-        while (live && k < total && nq < 2) {
+        while (live && k < total && nq < (converging() ? 1 : 2)) {
           const unsigned b = boundary(k);
           launch_chunk(int(b - k), gap && (b % gap_every) == 0, use_graph);
           const int q = next;
@@ -1183,6 +1193,11 @@ struct Engine {
         std::memcpy(h_ctl, h_ctl_q[q], CTL_WORDS * sizeof(int));
         std::memcpy(h_stats, h_stats_q[q], ST_WORDS * sizeof(double));
         report();
+        if (h_ctl[CTL_NGAP] > seen_ngap) {
+          seen_ngap = h_ctl[CTL_NGAP];
+          prev_rel = last_rel;
+          last_rel = h_stats[ST_REL];
+        }
         if (!h_ctl[CTL_ACTIVE]) live = false;
       }
     } else {
