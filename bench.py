@@ -166,6 +166,13 @@ def _timed_solves(opt, u0_dev, steps: int) -> float:
     return float(np.median(ts))
 
 
+def _kernels(prof) -> dict:
+    """Per-launch duration (ms), algorithmic bytes (MB) and the GB/s they
+    give, per kernel of one iteration (pdt_solver_profile)."""
+    return {p["name"]: {"ms": round(p["ms"], 5), "alg_MB": round(p["bytes"] / 1e6, 3),
+                        "GBs": round(p["bytes"] / (p["ms"] * 1e-3) / 1e9, 1)} for p in prof}
+
+
 def extra_tv256(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
     """ROF denoising, 256^2 float32, lambda 10, 500 iterations.  Accuracy is
     the relative L2 distance to the FP64 GPU solver on the same input."""
@@ -187,7 +194,7 @@ def extra_tv256(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
     out = {"workload": "TV-L2 (ROF) 256x256 float32, lambda=10, 500 iterations",
            "seconds": s, "us_per_iteration": s * 1e6 / 500, "rel_gap": float(st["rel_gap"]),
            "rel_l2_vs_f64": _rel(u, res["f64"][2]), "seconds_f64": res["f64"][1],
-           "kernels": {k["name"]: round(k["ms"], 5) for k in opt.profileKernels(50)}}
+           "kernels": _kernels(opt.profileKernels(50))}
     if not no_cpu:
         O = _oracle()
         O.set_threads(cpu_threads)
@@ -220,7 +227,7 @@ def extra_tgv512(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
         opt, s, u = res[name]
         out[name] = {"seconds": s, "us_per_iteration": s * 1e6 / 1000, "rel_l2_vs_f64": _rel(u, ref),
                      "rel_gap": float(opt.getStats()["rel_gap"]),
-                     "kernels": {k["name"]: round(k["ms"], 5) for k in opt.profileKernels(50)}}
+                     "kernels": _kernels(opt.profileKernels(50))}
     out["seconds_f64"] = res["f64"][1]
     if not no_cpu:
         O = _oracle()
@@ -234,7 +241,7 @@ def extra_tgv512(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
     return out
 
 
-def extra_demo384(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
+def extra_demo384(steps: int, cpu_threads: int, no_cpu: bool, autotune: bool = True, graph: bool = True) -> dict:
     """The geometry of the reference's Cartesian demo (bin/mri_cartesian_test.py):
     384x384 image on the 768 FE x 384 PE oversampled grid, the reference's
     random4 mask (117 PE lines, as stored in tests/golden/masks.npz), 15
@@ -247,8 +254,9 @@ def extra_demo384(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
     a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
     import primaldualtoolbox as pdt
     op, opt = build_solver(case, a1)
+    opt.getParameters().use_graph = int(graph)
     u0 = C.DeviceArray.from_numpy(op.adjoint(case["kspace"]), np.complex64)
-    tiles = pdt.mri.autotune([opt], reps=10)
+    tiles = pdt.mri.autotune([opt], reps=10) if autotune else None  # else the cached tiles
     opt.setNoisyData(case["kspace"])
     opt.solve()  # first solve: engine, graphs
     s = _timed_solves(opt, u0, steps)
@@ -263,7 +271,7 @@ def extra_demo384(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
                         "mask (117 PE lines), 30 dB SNR, alpha1=1e-5||f||, alpha0=2 alpha1, cold start"),
            "seconds_to_gap": s, "iterations_to_gap": it, "us_per_iteration": s * 1e6 / it,
            "seconds_1001_iterations": s_fixed, "tiles": tiles,
-           "kernels": {k["name"]: round(k["ms"], 5) for k in opt.profileKernels(50)}}
+           "kernels": _kernels(opt.profileKernels(50))}
     if not no_cpu:
         O = _oracle()
         O.set_threads(cpu_threads)
@@ -277,7 +285,8 @@ def extra_demo384(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
     return out
 
 
-def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool) -> dict:
+def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool, autotune: bool = True,
+                  graph: bool = True) -> dict:
     """`count` mixed Cartesian TGV-MRI slices (sizes 256^2..384x320, 8-20
     coils, random or regular masks) solved together to rel. gap 1e-4 in one
     batched solve; reports slices/s and per-slice iterations."""
@@ -299,6 +308,7 @@ def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool) -> d
         prm = opt.getParameters()
         prm.alpha1, prm.alpha0, prm.reduction, prm.max_iter = a1, 2 * a1, 1.0, 200000
         prm.gap_tol, prm.gap_every = GAP_TOL, GAP_EVERY
+        prm.use_graph = int(graph)
         f_dev = C.DeviceArray.from_numpy(case["kspace"], np.complex64)
         H, W = case["image"].shape
         u0 = C.DeviceArray.empty(C.PDT_C64, [W, H])
@@ -307,7 +317,7 @@ def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool) -> d
         u0s.append(u0)
         a1s.append(a1)
     solvers = [o for _, o in opts]
-    tune = pdt.mri.autotune(solvers, reps=5)
+    tune = pdt.mri.autotune(solvers, reps=5) if autotune else None  # else the cached tiles
     log(f"[bench] batch: tiles {tune}")
 
     def cold():
@@ -367,6 +377,33 @@ def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool) -> d
     return out
 
 
+def side_measurements(args, keys=None) -> dict:
+    """The side measurements named in keys (all when None): cnn, tv_256,
+    tgv_512, demo_384, batch (the batch of args.batch slices)."""
+    runs = {"cnn": extra_cnn,
+            "tv_256": lambda: extra_tv256(3, args.cpu_threads, args.no_cpu),
+            "tgv_512": lambda: extra_tgv512(3, args.cpu_threads, args.no_cpu),
+            "demo_384": lambda: extra_demo384(3, args.cpu_threads, args.no_cpu, not args.no_autotune,
+                                              not args.no_graph),
+            "batch": lambda: extra_batch64(args.batch, args.gen_workers, args.cpu_threads, args.no_cpu,
+                                           not args.no_autotune, not args.no_graph)}
+    unknown = set(keys or ()) - set(runs)
+    if unknown:
+        raise SystemExit(f"unknown side measurement(s): {sorted(unknown)}; known: {sorted(runs)}")
+    extra = {}
+    for key, fn in runs.items():
+        if keys is not None and key not in keys:
+            continue
+        name = f"batch_{args.batch}" if key == "batch" else key
+        log(f"[bench] side measurement {name}")
+        try:
+            extra[name] = fn()
+        except Exception as e:  # reported, never hides the headline line
+            extra[name] = {"error": f"{type(e).__name__}: {e}"}
+        log(f"[bench] {name}: {extra[name]}")
+    return extra
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -387,6 +424,12 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="slices in the batched side measurement")
     ap.add_argument("--gen-workers", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-warm", action="store_true", help="report the cold start only")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="side measurements launch their iterations directly instead of replaying graphs "
+                         "(for profilers that do not follow graph launches)")
+    ap.add_argument("--only", default="",
+                    help="comma-separated side measurements to run alone (cnn, tv_256, tgv_512, demo_384, "
+                         "batch), printed as one JSON line {\"extra\": ...} instead of the headline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -403,6 +446,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         log(f"[bench] copied {args.calibrate_copy} MiB x 5")
+        return
+
+    if args.only:
+        print(json.dumps({"extra": side_measurements(args, args.only.split(","))}), flush=True)
         return
 
     from primaldualtoolbox import _capi as C
@@ -480,21 +527,7 @@ def main():
         log(f"[bench] CPU baseline: {args.cpu_iters} iterations on {args.cpu_threads} threads")
         cpu = cpu_baseline(case, a1, int(np.median(iters)), args.cpu_iters, args.cpu_threads)
 
-    extra = None
-    if not args.no_extra:
-        extra = {}
-        for key, fn in (("cnn", extra_cnn),
-                        ("tv_256", lambda: extra_tv256(3, args.cpu_threads, args.no_cpu)),
-                        ("tgv_512", lambda: extra_tgv512(3, args.cpu_threads, args.no_cpu)),
-                        ("demo_384", lambda: extra_demo384(3, args.cpu_threads, args.no_cpu)),
-                        (f"batch_{args.batch}", lambda: extra_batch64(args.batch, args.gen_workers,
-                                                                      args.cpu_threads, args.no_cpu))):
-            log(f"[bench] side measurement {key}")
-            try:
-                extra[key] = fn()
-            except Exception as e:  # reported, never hides the headline line
-                extra[key] = {"error": f"{type(e).__name__}: {e}"}
-            log(f"[bench] {key}: {extra[key]}")
+    extra = None if args.no_extra else side_measurements(args)
 
     line = {
         "metric": "time-to-gap 1e-4 (s), 15-coil 320^2 Cartesian TGV, warm vs cold",
