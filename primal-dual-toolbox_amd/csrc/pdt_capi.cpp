@@ -981,6 +981,12 @@ struct Engine {
         // wide blocks for the adjoint pass when an item holds many coil rows
         x.adj_threads = x.cb_a * d0.W >= 8 * kRowThreads ? kRowAdjThreadsWide : kRowThreads;
         x.lds_rows_a = lds_rows_bytes(dt, x.cb_a, d0.W, nc, x.adj_threads != kRowAdjThreadsWide);
+      } else {
+        // one thread per pixel of an item where the workgroup size allows
+        int maxW = 1;
+        for (int i : groups[r.g]) maxW = std::max(maxW, hs[i].W);
+        const int px = maxW * L.rpi;
+        x.den_threads = px > 512 ? 1024 : px > 256 ? 512 : kBlock;
       }
       if (sv.size() == 1) {
         auto identity = [](const std::vector<int2>& v, int step) {

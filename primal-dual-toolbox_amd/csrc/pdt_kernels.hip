@@ -790,8 +790,8 @@ __device__ __forceinline__ bool slice_active(const SliceDev& d) { return gptr<co
 // ---------------------------------------------------------------------------
 
 // TV_dual_p_kernel (tgv_kernels.cuh:294-324).  Gap: P(ubar) terms.
-template <class P, bool GAP>
-__global__ __launch_bounds__(kBlock) void k_tv_dual(const SliceDev* sl, const int2* items, int rpi,
+template <class P, bool GAP, int NT>
+__global__ __launch_bounds__(NT) void k_tv_dual(const SliceDev* sl, const int2* items, int rpi,
                                                     double* part) {
   const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
   const SliceDev& d = sl[it.x];
@@ -804,7 +804,7 @@ __global__ __launch_bounds__(kBlock) void k_tv_dual(const SliceDev* sl, const in
   const auto f = gptr<const P>(d.f);
   double t[kGapTerms] = {0};
   const int cnt = (yb - ya) * W;
-  for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
+  for (int idx = threadIdx.x; idx < cnt; idx += NT) {
     const int yy = idx / W;
     const int x = idx - yy * W, y = ya + yy;
     dual_pixel<P, P, false, GAP>(d, x, y, sigma, R(0), R(0), t);
@@ -813,13 +813,13 @@ __global__ __launch_bounds__(kBlock) void k_tv_dual(const SliceDev* sl, const in
       t[3] += dabs2(ub[i] - f[i]);
     }
   }
-  if (GAP) block_sum<4>(reinterpret_cast<double(&)[4]>(t), part + size_t(blockIdx.x) * kGapTerms);
+  if (GAP) block_sum<4, NT>(reinterpret_cast<double(&)[4]>(t), part + size_t(blockIdx.x) * kGapTerms);
 }
 
 // TV_primal_u_kernel 2D (tgv_kernels.cuh:568-605).  Gap: ROF dual terms
 // Re<div p, f>, ||div p||^2.
-template <class P, bool GAP>
-__global__ __launch_bounds__(kBlock) void k_tv_primal(const SliceDev* sl, const int2* items, int rpi,
+template <class P, bool GAP, int NT>
+__global__ __launch_bounds__(NT) void k_tv_primal(const SliceDev* sl, const int2* items, int rpi,
                                                       double* part) {
   const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
   const SliceDev& d = sl[it.x];
@@ -836,7 +836,7 @@ __global__ __launch_bounds__(kBlock) void k_tv_primal(const SliceDev* sl, const 
   const auto py = px + n;
   double t[2] = {0, 0};
   const int cnt = (yb - ya) * W;
-  for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
+  for (int idx = threadIdx.x; idx < cnt; idx += NT) {
     const int yy = idx / W;
     const int x = idx - yy * W, y = ya + yy;
     const size_t i = size_t(y) * W + x;
@@ -851,14 +851,14 @@ __global__ __launch_bounds__(kBlock) void k_tv_primal(const SliceDev* sl, const 
       t[1] += dabs2(div);
     }
   }
-  if (GAP) block_sum<2>(t, part + size_t(blockIdx.x) * kGapTerms);
+  if (GAP) block_sum<2, NT>(t, part + size_t(blockIdx.x) * kGapTerms);
   if (ya == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += 1;
 }
 
 // TGV denoising primal half (tgvoptimizer.cu:119-127): TV_primal_u_kernel
 // then TGV_primal_v_kernel at every pixel.
-template <class P, class S>
-__global__ __launch_bounds__(kBlock) void k_tgv_primal(const SliceDev* sl, const int2* items, int rpi) {
+template <class P, class S, int NT>
+__global__ __launch_bounds__(NT) void k_tgv_primal(const SliceDev* sl, const int2* items, int rpi) {
   const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
@@ -873,7 +873,7 @@ __global__ __launch_bounds__(kBlock) void k_tgv_primal(const SliceDev* sl, const
   const auto px = gptr<const S>(d.p);
   const auto py = px + n;
   const int cnt = (yb - ya) * W;
-  for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
+  for (int idx = threadIdx.x; idx < cnt; idx += NT) {
     const int yy = idx / W;
     const int x = idx - yy * W, y = ya + yy;
     const size_t i = size_t(y) * W + x;
@@ -888,8 +888,8 @@ __global__ __launch_bounds__(kBlock) void k_tgv_primal(const SliceDev* sl, const
 }
 
 // TGV denoising dual half (tgvoptimizer.cu:130-136).  Gap: P(ubar, vbar).
-template <class P, class S, bool GAP>
-__global__ __launch_bounds__(kBlock) void k_tgv_dual(const SliceDev* sl, const int2* items, int rpi,
+template <class P, class S, bool GAP, int NT>
+__global__ __launch_bounds__(NT) void k_tgv_dual(const SliceDev* sl, const int2* items, int rpi,
                                                      double* part) {
   const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
   const SliceDev& d = sl[it.x];
@@ -902,7 +902,7 @@ __global__ __launch_bounds__(kBlock) void k_tgv_dual(const SliceDev* sl, const i
   const auto f = gptr<const P>(d.f);
   double t[kGapTerms] = {0};
   const int cnt = (yb - ya) * W;
-  for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
+  for (int idx = threadIdx.x; idx < cnt; idx += NT) {
     const int yy = idx / W;
     const int x = idx - yy * W, y = ya + yy;
     dual_pixel<P, S, true, GAP>(d, x, y, sigma, rcp(a0), rcp(a1), t);
@@ -911,7 +911,7 @@ __global__ __launch_bounds__(kBlock) void k_tgv_dual(const SliceDev* sl, const i
       t[3] += dabs2(ub[i] - f[i]);
     }
   }
-  if (GAP) block_sum<4>(reinterpret_cast<double(&)[4]>(t), part + size_t(blockIdx.x) * kGapTerms);
+  if (GAP) block_sum<4, NT>(reinterpret_cast<double(&)[4]>(t), part + size_t(blockIdx.x) * kGapTerms);
   if (ya == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += 1;
 }
 
@@ -1723,24 +1723,32 @@ template <class P, class S>
 static void denoise_stage(const Launch& L, int kind, bool gap, int stage) {
   const int n = L.nrows;
   if (n == 0) return;
-  if (kind == K_TV) {
-    if (stage == 0) {
-      if (gap) PDT_LAUNCH((k_tv_dual<P, true>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[0]);
-      else PDT_LAUNCH((k_tv_dual<P, false>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[0]);
-    } else {
-      if (gap) PDT_LAUNCH((k_tv_primal<P, true>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[1]);
-      else PDT_LAUNCH((k_tv_primal<P, false>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[1]);
-    }
-  } else {
-    if (stage == 0) {
-      PDT_LAUNCH((k_tgv_primal<P, S>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi);
+  const int2* rows = L.dense ? nullptr : L.rows;
+  // the half-iteration kernels with L.den_threads threads per workgroup
+  auto launch = [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    if (kind == K_TV) {
+      if (stage == 0) {
+        if (gap) hipLaunchKernelGGL((k_tv_dual<P, true, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi, L.part[0]);
+        else hipLaunchKernelGGL((k_tv_dual<P, false, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi, L.part[0]);
+      } else {
+        if (gap) hipLaunchKernelGGL((k_tv_primal<P, true, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi, L.part[1]);
+        else hipLaunchKernelGGL((k_tv_primal<P, false, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi, L.part[1]);
+      }
+    } else if (stage == 0) {
+      hipLaunchKernelGGL((k_tgv_primal<P, S, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi);
     } else if (gap) {
-      PDT_LAUNCH((k_tgv_dual<P, S, true>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[0]);
-      PDT_LAUNCH((k_tgv_restore1<P, S>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[2]);
-      PDT_LAUNCH((k_tgv_restore2<P>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[3]);
+      hipLaunchKernelGGL((k_tgv_dual<P, S, true, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi, L.part[0]);
     } else {
-      PDT_LAUNCH((k_tgv_dual<P, S, false>), n, 0, L.stream, L.sl, L.dense ? nullptr : L.rows, L.rpi, L.part[0]);
+      hipLaunchKernelGGL((k_tgv_dual<P, S, false, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi, L.part[0]);
     }
+  };
+  if (L.den_threads >= 1024) launch(std::integral_constant<int, 1024>{});
+  else if (L.den_threads >= 512) launch(std::integral_constant<int, 512>{});
+  else launch(std::integral_constant<int, kBlock>{});
+  if (kind == K_TGV && stage == 1 && gap) {
+    PDT_LAUNCH((k_tgv_restore1<P, S>), n, 0, L.stream, L.sl, rows, L.rpi, L.part[2]);
+    PDT_LAUNCH((k_tgv_restore2<P>), n, 0, L.stream, L.sl, rows, L.rpi, L.part[3]);
   }
 }
 

@@ -42,6 +42,9 @@ struct Launch {
   // the denoising rows), so the kernels take their item from the block
   // index (no list load)
   bool dense = false;
+  // workgroup size of the denoising half-iterations (256, 512 or 1024): the
+  // smallest that covers rpi rows of the widest slice in one pass, at most 1024
+  int den_threads = 256;
 };
 
 // Block size of k_mri_primal covering rows of width W in one pass.
