@@ -872,7 +872,9 @@ struct Engine {
     d_rg.alloc(std::max<size_t>(1, rgs.size()) * sizeof(int2));
     d_rg_a.alloc(std::max<size_t>(1, rgs_a.size()) * sizeof(int2));
     d_cols.alloc(std::max<size_t>(1, cols.size()) * sizeof(int2));
-    const size_t nitem = std::max<size_t>(1, std::max({rows.size(), rgs.size(), rgs_a.size(), cols.size()}));
+    // (a single slice's dual-step workgroups write after its row items)
+    const size_t rg_items = rgs.size() + (mri && ns == 1 ? size_t(dual_groups(hs[0])) : 0);
+    const size_t nitem = std::max<size_t>(1, std::max({rows.size(), rg_items, rgs_a.size(), cols.size()}));
     d_part.alloc(4 * nitem * kGapTerms * sizeof(double));
     L.stream = stream;
     L.sl = d_slices.as<SliceDev>();
@@ -942,8 +944,24 @@ struct Engine {
   // Copies slices and lists to the device (the stream is idle) and derives
   // one launch per non-empty group: its sub-ranges of the lists and of the
   // partial-sum buffers, its FFT lengths and LDS sizes.
+  // Workgroups of k_rows_fwd that take a single slice's dual step off its
+  // row items: two pixels per thread.
+  static int dual_groups(const SliceDev& d) { return (d.W * d.H + 2 * kRowThreads - 1) / (2 * kRowThreads); }
+
   void upload_lists(const std::vector<int2>& rows, const std::vector<int2>& rgs, const std::vector<int2>& rgs_a,
                     const std::vector<int2>& cols, const std::vector<GroupRange>& gr) {
+    auto identity = [](const std::vector<int2>& v, int step) {
+      for (size_t i = 0; i < v.size(); ++i)
+        if (v[i].x != 0 || v[i].y != int(i) * step) return false;
+      return true;
+    };
+    const bool one = sv.size() == 1;
+    const bool dense = one && (sv[0]->is_op() ? identity(rgs, 1) && identity(rgs_a, 1) && identity(cols, 1)
+                                              : identity(rows, L.rpi));
+    // a dense single MRI slice hands its dual step to separate workgroups,
+    // whose gap terms follow the row items'
+    const int ndual = dense && sv[0]->is_op() ? dual_groups(hs[0]) : 0;
+    if (one && sv[0]->is_op()) hs[0].rg_end = hs[0].rg_beg + (int)rgs.size() + ndual;
     HIPC(hipMemcpy(d_slices.p, hs.data(), hs.size() * sizeof(SliceDev), hipMemcpyHostToDevice));
     if (!rows.empty()) HIPC(hipMemcpy(d_rows.p, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
     if (!rgs.empty()) HIPC(hipMemcpy(d_rg.p, rgs.data(), rgs.size() * sizeof(int2), hipMemcpyHostToDevice));
@@ -988,14 +1006,8 @@ struct Engine {
         const int px = maxW * L.rpi;
         x.den_threads = px > 512 ? 1024 : px > 256 ? 512 : kBlock;
       }
-      if (sv.size() == 1) {
-        auto identity = [](const std::vector<int2>& v, int step) {
-          for (size_t i = 0; i < v.size(); ++i)
-            if (v[i].x != 0 || v[i].y != int(i) * step) return false;
-          return true;
-        };
-        x.dense = mri ? identity(rgs, 1) && identity(rgs_a, 1) && identity(cols, 1) : identity(rows, L.rpi);
-      }
+      x.dense = dense;
+      x.ndual = ndual;
       listed_items += r.nrow + r.nrg + r.nrga + r.ncol;
       G.push_back(x);
     }

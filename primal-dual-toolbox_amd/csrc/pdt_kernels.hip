@@ -1038,17 +1038,40 @@ __device__ __forceinline__ void copy_to_lds(T* dst, A* src, int n) {
 // each on its own range of pixels.  Then for
 // each coil of the group: S_c * src row -> ifftshift -> PE-FFT -> fftshift
 // -> sampled columns -> T1 [c][ks][y].
+// With ndual > 0 (a single slice, dense items) the dual step is left to
+// ndual further workgroups after the nitems row items, each on a contiguous
+// range of pixels (gap terms at part[nitems + b]), so that the FFT chains of
+// the row items do not wait for it.
 // The gap-evaluation variant holds extra FP64 accumulators; it is asked to
 // fit five waves per SIMD, as many as the plain variant, so that the same
 // tile shapes keep every block resident.
 template <class R, int MODE, bool GAP, int NT, int NF>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GAP ? 5 : 1))) void k_rows_fwd(const SliceDev* sl, const int2* items, int nitems, int cb,
-                                                 double* part) {
+                                                 double* part, int ndual) {
   extern __shared__ char smem_raw[];
+  using C = Cx<R>;
+  if (MODE != M_OP && int(blockIdx.x) >= nitems) {
+    const SliceDev& d = sl[0];
+    if (!slice_active(d)) return;
+    const int b = int(blockIdx.x) - nitems, W = d.W, WH = W * d.H;
+    const int chunk = (WH + ndual - 1) / ndual;
+    const int pa = min(WH, b * chunk), pe = min(WH, pa + chunk);
+    const int k = gptr<int>(d.ctl)[CTL_ITER];
+    const R sigma = R(d.sigma);
+    const R ia0 = rcp(sched_alpha<R>(d, d.alpha0, d.alpha0_end, k));
+    const R ia1 = rcp(sched_alpha<R>(d, d.alpha1, d.alpha1_end, k));
+    const float rW = 1.0f / float(W);
+    double t[kGapTerms] = {0};
+    for (int i = pa + threadIdx.x; i < pe; i += NT) {
+      const int y = qdiv(i, W, rW), x = i - y * W;
+      dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, ia0, ia1, t);
+    }
+    if (GAP) block_sum<3, NT>(reinterpret_cast<double(&)[3]>(t), part + size_t(nitems + b) * kGapTerms);
+    return;
+  }
   const int li = xcd_item(blockIdx.x, nitems);
   const int2 it = items ? items[li] : make_int2(0, li);  // null: the dense single-slice list
   const SliceDev& d = sl[it.x];
-  using C = Cx<R>;
   const int W = NF > 0 ? NF : d.W, H = d.H, nc = d.ncols;
   const int y = it.y / d.ngr, g = it.y - y * d.ngr;
   const int c0 = g * cb, ncb = min(cb, d.C - c0);
@@ -1096,7 +1119,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GAP ? 5 : 1)
   // after the loads above are issued, so its control-word load overlaps them.
   if (!slice_active(d)) return;
   double t[kGapTerms] = {0};
-  if (MODE != M_OP) {
+  if (MODE != M_OP && ndual == 0) {
     // The dual step of row y is shared out over the row's coil groups in
     // pixel ranges of a multiple of 32, so no group carries all of it.
     const int chunk = ((W + d.ngr - 1) / d.ngr + 31) & ~31;
@@ -1801,8 +1824,9 @@ static void mri_stage_m(const Launch& L, bool gap, int stage) {
   if (stage == 0) {
     with_fft_len<R>(L.fft_w, [&](auto nf) {
       constexpr int NF = decltype(nf)::value;
-      if (gap) PDT_LAUNCH_NT((k_rows_fwd<R, MODE, true, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.dense ? nullptr : L.rowgroups, L.nrg, L.cb, L.part[0]);
-      else PDT_LAUNCH_NT((k_rows_fwd<R, MODE, false, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.dense ? nullptr : L.rowgroups, L.nrg, L.cb, L.part[0]);
+      const int nd = L.dense ? L.ndual : 0;
+      if (gap) PDT_LAUNCH_NT((k_rows_fwd<R, MODE, true, NR, NF>), L.nrg + nd, NR, L.lds_rows, L.stream, L.sl, L.dense ? nullptr : L.rowgroups, L.nrg, L.cb, L.part[0], nd);
+      else PDT_LAUNCH_NT((k_rows_fwd<R, MODE, false, NR, NF>), L.nrg + nd, NR, L.lds_rows, L.stream, L.sl, L.dense ? nullptr : L.rowgroups, L.nrg, L.cb, L.part[0], nd);
     });
   } else if (stage == 1) {
     if (L.ncolitems == 0) return;
@@ -1865,7 +1889,7 @@ static void op_forward(const Launch& L) {
   constexpr int NR = kRowThreads, NC = kColThreads;
   with_fft_len<R>(L.fft_w, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;
-    PDT_LAUNCH_NT((k_rows_fwd<R, M_OP, false, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0]);
+    PDT_LAUNCH_NT((k_rows_fwd<R, M_OP, false, NR, NF>), L.nrg, NR, L.lds_rows, L.stream, L.sl, L.rowgroups, L.nrg, L.cb, L.part[0], 0);
   });
   with_fft_len<R>(L.fft_h, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;

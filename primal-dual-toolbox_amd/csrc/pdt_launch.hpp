@@ -42,6 +42,9 @@ struct Launch {
   // the denoising rows), so the kernels take their item from the block
   // index (no list load)
   bool dense = false;
+  // single dense slice: workgroups of k_rows_fwd that do the dual step alone
+  // after the row items (0: the row items share it out)
+  int ndual = 0;
   // workgroup size of the denoising half-iterations (256, 512 or 1024): the
   // smallest that covers rpi rows of the widest slice in one pass, at most 1024
   int den_threads = 256;

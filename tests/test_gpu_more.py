@@ -173,7 +173,9 @@ def _batch_pair(pdt, specs):
 def test_batch_ragged_stops_equal_single_solves():
     """Slices of one batch that stop at different iterations (iteration
     limits and gap tolerances) leave the work lists as they finish; every
-    slice's result, iteration count and gap equal those of its own solve."""
+    slice's result and iteration count equal those of its own solve, and its
+    gap to 1e-12 (a single solve sums the dual step's FP64 gap terms over
+    other workgroups than a batch does)."""
     pdt = _pdt()
     specs = [(32, 32, 3, 1, 49, 0.0), (24, 24, 2, 2, 299, 0.0), (32, 32, 4, 3, 999, 1e-2),
              (20, 28, 2, 4, 149, 0.0), (32, 32, 3, 5, 999, 3e-3), (24, 24, 2, 6, 99, 0.0)]
@@ -185,7 +187,7 @@ def test_batch_ragged_stops_equal_single_solves():
         sa, sb = a.getStats(), b.getStats()
         print(f"  iters single {sa['iters']} batch {sb['iters']}", flush=True)
         assert sa["iters"] == sb["iters"]
-        assert sa["rel_gap"] == sb["rel_gap"]
+        assert abs(sa["rel_gap"] - sb["rel_gap"]) <= 1e-12 * abs(sb["rel_gap"])
         assert np.array_equal(a.getResult(), b.getResult())
 
 
