@@ -158,6 +158,31 @@ def test_odd_radix_split_over_waves(oracle, cb, cg, cb_a):
     assert err < 1e-3
 
 
+@pytest.mark.parametrize("n_pe,coils", [(256, 8), (384, 6), (512, 4), (640, 4)])
+def test_wide_adjoint_permuted_slots(oracle, n_pe, coils):
+    """Every coil of a row in one 1024-thread adjoint workgroup (cb_a = C,
+    cb_a * W >= 2048) for each row length whose wide FFT buffers have
+    XOR-permuted slots: 256, 384 and 512 with the register coil sum and the
+    fused primal step, 640 (2 W > 1024 threads) without them.  64 FE rows on
+    the 128 ROOS grid, 10 iterations vs the FP64 oracle."""
+    pdt = _pdt()
+    from primaldualtoolbox import synthetic as syn
+    case = syn.mri_case(seed=12, n=64, n_pe=n_pe, coils=coils)
+    a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
+    _, opt = _mri_solver(pdt, case, a1=a1, max_iter=9)
+    pdt.mri.set_tiles([opt], cb=4, cg=1, cb_a=coils)
+    try:
+        opt.solve()
+    finally:
+        pdt.mri.set_tiles([opt])
+    oop = oracle.Op(oracle.OP_CARTESIAN_ROOS, case["sens"].astype(np.complex128), case["mask"].astype(np.float64))
+    ref, _, _, _ = oracle.solve(oracle.TGV_MRI, case["kspace"].astype(np.complex128), op=oop, alpha1=a1,
+                                alpha0=2 * a1, max_iter=9)
+    err = rel_l2(opt.getResult(), ref)
+    print(f"wide adjoint PE {n_pe}, {coils} coils: rel L2 {err:.3e}", flush=True)
+    assert err < 1e-3
+
+
 def _batch_pair(pdt, specs):
     """Two identical lists of configured solvers (for single vs batched)."""
     from primaldualtoolbox import synthetic as syn
