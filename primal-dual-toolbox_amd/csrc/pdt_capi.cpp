@@ -688,13 +688,19 @@ struct Engine {
   TuneKey key{};
   std::vector<pdt_solver*> sv;
   hipStream_t stream = nullptr;
-  DevBuf d_slices, d_rows, d_rg, d_rg_a, d_cols, d_part, d_ctl, d_stats;
-  int* h_ctl = nullptr;       // pinned mirrors
+  DevBuf d_slices, d_rows, d_rg, d_rg_a, d_cols, d_part;
+  // statistics (ns x ST_WORDS doubles) then control words (ns x CTL_WORDS
+  // ints) in one device block, so one copy brings both to the host
+  DevBuf d_cs;
+  double* d_stats = nullptr;
+  int* d_ctl = nullptr;
+  size_t cs_bytes = 0;
+  char* h_cs = nullptr;  // pinned mirror of d_cs
+  int* h_ctl = nullptr;
   double* h_stats = nullptr;
-  // single-slice pipelining: two pinned copies of the control words and
-  // statistics (one per chunk in flight) and their completion events
-  int* h_ctl_q[2] = {nullptr, nullptr};
-  double* h_stats_q[2] = {nullptr, nullptr};
+  // single-slice pipelining: two pinned copies of the block (one per chunk
+  // in flight) and their completion events
+  char* h_cs_q[2] = {nullptr, nullptr};
   hipEvent_t ev_q[2] = {nullptr, nullptr};
   int nrows = 0, nrg = 0, nrg_a = 0, ncolitems = 0;
   Launch L;                             // every slice: gap reduction, full-size buffers
@@ -711,11 +717,9 @@ struct Engine {
 
   ~Engine() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
-    if (h_ctl) (void)hipHostFree(h_ctl);
-    if (h_stats) (void)hipHostFree(h_stats);
+    if (h_cs) (void)hipHostFree(h_cs);
     for (int q = 0; q < 2; ++q) {
-      if (h_ctl_q[q]) (void)hipHostFree(h_ctl_q[q]);
-      if (h_stats_q[q]) (void)hipHostFree(h_stats_q[q]);
+      if (h_cs_q[q]) (void)hipHostFree(h_cs_q[q]);
       if (ev_q[q]) (void)hipEventDestroy(ev_q[q]);
     }
   }
@@ -859,13 +863,16 @@ struct Engine {
     const int tiles[5] = {cb, cg, rpi, int(sv[0]->prm.gap_every), cb_a};
     sig.append(reinterpret_cast<const char*>(tiles), sizeof(tiles));
     if (plan_only) return;
-    d_ctl.alloc(size_t(ns) * CTL_WORDS * sizeof(int));
-    d_stats.alloc(size_t(ns) * ST_WORDS * sizeof(double));
-    HIPC(hipHostMalloc((void**)&h_ctl, size_t(ns) * CTL_WORDS * sizeof(int), hipHostMallocDefault));
-    HIPC(hipHostMalloc((void**)&h_stats, size_t(ns) * ST_WORDS * sizeof(double), hipHostMallocDefault));
+    cs_bytes = size_t(ns) * (ST_WORDS * sizeof(double) + CTL_WORDS * sizeof(int));
+    d_cs.alloc(cs_bytes);
+    d_stats = d_cs.as<double>();
+    d_ctl = reinterpret_cast<int*>(d_stats + size_t(ns) * ST_WORDS);
+    HIPC(hipHostMalloc((void**)&h_cs, cs_bytes, hipHostMallocDefault));
+    h_stats = reinterpret_cast<double*>(h_cs);
+    h_ctl = reinterpret_cast<int*>(h_stats + size_t(ns) * ST_WORDS);
     for (int i = 0; i < ns; ++i) {
-      hs[i].ctl = d_ctl.as<int>() + size_t(i) * CTL_WORDS;
-      hs[i].stats = d_stats.as<double>() + size_t(i) * ST_WORDS;
+      hs[i].ctl = d_ctl + size_t(i) * CTL_WORDS;
+      hs[i].stats = d_stats + size_t(i) * ST_WORDS;
     }
     d_slices.alloc(hs.size() * sizeof(SliceDev));
     d_rows.alloc(std::max<size_t>(1, rows.size()) * sizeof(int2));
@@ -1087,9 +1094,8 @@ struct Engine {
       c[CTL_RUPD] = sv[i]->rupd;
       c[CTL_DONE_AT] = -1;
     }
-    HIPC(hipMemcpyAsync(d_ctl.p, h_ctl, size_t(ns) * CTL_WORDS * sizeof(int), hipMemcpyHostToDevice, stream));
     std::fill(h_stats, h_stats + size_t(ns) * ST_WORDS, 0.0);
-    HIPC(hipMemcpyAsync(d_stats.p, h_stats, size_t(ns) * ST_WORDS * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPC(hipMemcpyAsync(d_cs.p, h_cs, cs_bytes, hipMemcpyHostToDevice, stream));
   }
 
   // Enqueues `len` iterations; the last one evaluates the gap when asked.
@@ -1102,13 +1108,9 @@ struct Engine {
     }
   }
 
-  // Control words and statistics of every slice to pinned host memory
-  // (stream-ordered after the chunk enqueued before).
-  void copy_ctl(int* hc, double* hst) {
-    const int ns = (int)sv.size();
-    HIPC(hipMemcpyAsync(hc, d_ctl.p, size_t(ns) * CTL_WORDS * sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPC(hipMemcpyAsync(hst, d_stats.p, size_t(ns) * ST_WORDS * sizeof(double), hipMemcpyDeviceToHost, stream));
-  }
+  // Statistics and control words of every slice to pinned host memory laid
+  // out like d_cs, in one copy (stream-ordered after the chunk enqueued before).
+  void copy_ctl(char* host) { HIPC(hipMemcpyAsync(host, d_cs.p, cs_bytes, hipMemcpyDeviceToHost, stream)); }
 
   void launch_chunk(int len, bool gap, bool use_graph) {
     if (!use_graph) {
@@ -1184,8 +1186,7 @@ struct Engine {
         return last_rel * std::min(1.0, last_rel / prev_rel) <= 1.02 * tol;
       };
       for (int q = 0; q < 2; ++q) {
-        if (!h_ctl_q[q]) HIPC(hipHostMalloc((void**)&h_ctl_q[q], CTL_WORDS * sizeof(int), hipHostMallocDefault));
-        if (!h_stats_q[q]) HIPC(hipHostMalloc((void**)&h_stats_q[q], ST_WORDS * sizeof(double), hipHostMallocDefault));
+        if (!h_cs_q[q]) HIPC(hipHostMalloc((void**)&h_cs_q[q], cs_bytes, hipHostMallocDefault));
         if (!ev_q[q]) HIPC(hipEventCreateWithFlags(&ev_q[q], hipEventDisableTiming));
       }
       int queued[2], nq = 0, head = 0, next = 0;
@@ -1197,7 +1198,7 @@ struct Engine {
           launch_chunk(int(b - k), gap && (b % gap_every) == 0, use_graph);
           const int q = next;
           next ^= 1;
-          copy_ctl(h_ctl_q[q], h_stats_q[q]);
+          copy_ctl(h_cs_q[q]);
           HIPC(hipEventRecord(ev_q[q], stream));
           queued[(head + nq) % 2] = q;
           ++nq;
@@ -1208,8 +1209,7 @@ struct Engine {
         head = (head + 1) % 2;
         --nq;
         HIPC(hipEventSynchronize(ev_q[q]));
-        std::memcpy(h_ctl, h_ctl_q[q], CTL_WORDS * sizeof(int));
-        std::memcpy(h_stats, h_stats_q[q], ST_WORDS * sizeof(double));
+        std::memcpy(h_cs, h_cs_q[q], cs_bytes);
         report();
         if (h_ctl[CTL_NGAP] > seen_ngap) {
           seen_ngap = h_ctl[CTL_NGAP];
@@ -1223,7 +1223,7 @@ struct Engine {
       while (k < total) {
         const unsigned b = boundary(k);
         launch_chunk(int(b - k), gap && (b % gap_every) == 0, use_graph);
-        copy_ctl(h_ctl, h_stats);
+        copy_ctl(h_cs);
         HIPC(hipStreamSynchronize(stream));
         k = b;
         // a slice stops at its own iteration limit (host-driven, so that no
@@ -1241,7 +1241,7 @@ struct Engine {
         report();
         if (!any) break;
         if (changed)
-          HIPC(hipMemcpyAsync(d_ctl.p, h_ctl, size_t(ns) * CTL_WORDS * sizeof(int), hipMemcpyHostToDevice, stream));
+          HIPC(hipMemcpyAsync(d_ctl, h_ctl, size_t(ns) * CTL_WORDS * sizeof(int), hipMemcpyHostToDevice, stream));
         compact_lists();
       }
     }

@@ -1561,8 +1561,10 @@ __global__ __launch_bounds__(1024) void k_mri_primal(const SliceDev* sl, const i
 // over the row items.  MRI: part[0] rows_fwd, part[2] rows_adj over the row
 // items, part[1] cols over the column items.
 // ---------------------------------------------------------------------------
+constexpr int kGapThreads = 1024;  // one workgroup per slice, loads of many items in flight
+
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_gap_reduce(const SliceDev* sl, int kind, const double* p0,
+__global__ __launch_bounds__(kGapThreads) void k_gap_reduce(const SliceDev* sl, int kind, const double* p0,
                                                        const double* p1, const double* p2,
                                                        const double* p3) {
   const SliceDev& d = sl[blockIdx.x];
@@ -1573,11 +1575,11 @@ __global__ __launch_bounds__(kBlock) void k_gap_reduce(const SliceDev* sl, int k
   if (mri) {
     // four items per step, their loads issued before the additions (which
     // keep the order of a plain strided loop)
-    for (int i0 = d.rg_beg + threadIdx.x; i0 < d.rg_end; i0 += 4 * kBlock) {
+    for (int i0 = d.rg_beg + threadIdx.x; i0 < d.rg_end; i0 += 4 * kGapThreads) {
       double a[4][3];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int i = i0 + u * kBlock;
+        const int i = i0 + u * kGapThreads;
         const bool in = i < d.rg_end;
         const double* q = p0 + size_t(in ? i : i0) * kGapTerms;
         a[u][0] = in ? q[0] : 0.0;
@@ -1592,7 +1594,7 @@ __global__ __launch_bounds__(kBlock) void k_gap_reduce(const SliceDev* sl, int k
       }
     }
   }
-  for (int i = d.row_beg + threadIdx.x; i < d.row_end; i += kBlock) {
+  for (int i = d.row_beg + threadIdx.x; i < d.row_end; i += kGapThreads) {
     const double* a = p0 + size_t(i) * kGapTerms;
     if (kind == K_TV) {
       s[0] += a[0];
@@ -1612,11 +1614,11 @@ __global__ __launch_bounds__(kBlock) void k_gap_reduce(const SliceDev* sl, int k
     }
   }
   if (mri) {
-    for (int i0 = d.col_beg + threadIdx.x; i0 < d.col_end; i0 += 4 * kBlock) {
+    for (int i0 = d.col_beg + threadIdx.x; i0 < d.col_end; i0 += 4 * kGapThreads) {
       double a[4][2];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int i = i0 + u * kBlock;
+        const int i = i0 + u * kGapThreads;
         const bool in = i < d.col_end;
         const double* q = p1 + size_t(in ? i : i0) * kGapTerms;
         a[u][0] = in ? q[0] : 0.0;
@@ -1630,8 +1632,8 @@ __global__ __launch_bounds__(kBlock) void k_gap_reduce(const SliceDev* sl, int k
     }
   }
   __shared__ double tot[kGapTerms];
-  block_sum<kGapTerms>(s, tot);
-  if (kind == K_TGV) mx = block_max(mx);
+  block_sum<kGapTerms, kGapThreads>(s, tot);
+  if (kind == K_TGV) mx = block_max<kGapThreads>(mx);
   __syncthreads();
   if (threadIdx.x != 0) return;
   const double lam = d.lambda;
@@ -1879,9 +1881,11 @@ void run_iteration(const Launch& L, int kind, int dtype, bool fp16, bool gap) {
 
 void run_gap_reduce(const Launch& L, int kind, int dtype) {
   if (dtype == DT_F32 || dtype == DT_C64)
-    PDT_LAUNCH((k_gap_reduce<float>), L.nsl, 0, L.stream, L.sl, kind, L.part[0], L.part[1], L.part[2], L.part[3]);
+    hipLaunchKernelGGL((k_gap_reduce<float>), dim3(L.nsl), dim3(kGapThreads), 0, L.stream, L.sl, kind, L.part[0],
+                       L.part[1], L.part[2], L.part[3]);
   else
-    PDT_LAUNCH((k_gap_reduce<double>), L.nsl, 0, L.stream, L.sl, kind, L.part[0], L.part[1], L.part[2], L.part[3]);
+    hipLaunchKernelGGL((k_gap_reduce<double>), dim3(L.nsl), dim3(kGapThreads), 0, L.stream, L.sl, kind, L.part[0],
+                       L.part[1], L.part[2], L.part[3]);
 }
 
 template <class R>
