@@ -52,6 +52,31 @@ def test_tv_denoise_matches_oracle(oracle, shape, cplx):
     assert opt.getStats()["iters"] == 200
 
 
+@pytest.mark.parametrize("W", [300, 700, 1100])
+def test_denoise_row_widths_match_oracle(oracle, W):
+    """Row widths for each denoising workgroup size: 300 (512 threads, some
+    idle), 700 (1024 threads), 1100 (1024 threads, two pixels for some);
+    TV and TGV (complex), 100 iterations vs the FP64 oracle."""
+    pdt = _pdt()
+    rng = _rng(9)
+    f = _crandn(rng, (6, W)).astype(np.complex64)
+    tv = pdt.denoising.TvOptimizer_2c()
+    tv.setNoisyData(f)
+    tv.getParameters().Lambda, tv.getParameters().max_iter = 10, 99
+    tv.solve()
+    ref, _, _, _ = oracle.solve(oracle.TV, f.astype(np.complex128), lambda_=10, max_iter=99)
+    e_tv = rel_l2(tv.getResult(), ref)
+    tgv = pdt.denoising.TgvOptimizer_2c()
+    tgv.setNoisyData(f)
+    prm = tgv.getParameters()
+    prm.Lambda, prm.alpha0, prm.alpha1, prm.max_iter = 20, 2, 1, 99
+    tgv.solve()
+    ref, _, _, _ = oracle.solve(oracle.TGV, f.astype(np.complex128), lambda_=20, alpha0=2, alpha1=1, max_iter=99)
+    e_tgv = rel_l2(tgv.getResult(), ref)
+    print(f"  W={W}: TV rel L2 {e_tv:.3e}, TGV rel L2 {e_tgv:.3e}", flush=True)
+    assert e_tv < TOL_U and e_tgv < TOL_U
+
+
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("fp16", [False, True])
 def test_tgv_denoise_matches_oracle(oracle, cplx, fp16):
