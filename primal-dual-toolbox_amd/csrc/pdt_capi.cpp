@@ -881,7 +881,9 @@ struct Engine {
     d_cols.alloc(std::max<size_t>(1, cols.size()) * sizeof(int2));
     // (a single slice's dual-step workgroups write after its row items)
     const size_t rg_items = rgs.size() + (mri && ns == 1 ? size_t(dual_groups(hs[0])) : 0);
-    const size_t nitem = std::max<size_t>(1, std::max({rows.size(), rg_items, rgs_a.size(), cols.size()}));
+    // (and its v-step workgroups after its rows)
+    const size_t row_items = rows.size() + (mri && ns == 1 ? size_t(vstep_groups(hs[0])) : 0);
+    const size_t nitem = std::max<size_t>(1, std::max({row_items, rg_items, rgs_a.size(), cols.size()}));
     d_part.alloc(4 * nitem * kGapTerms * sizeof(double));
     L.stream = stream;
     L.sl = d_slices.as<SliceDev>();
@@ -954,6 +956,9 @@ struct Engine {
   // Workgroups of k_rows_fwd that take a single slice's dual step off its
   // row items: two pixels per thread.
   static int dual_groups(const SliceDev& d) { return (d.W * d.H + 2 * kRowThreads - 1) / (2 * kRowThreads); }
+  // Workgroups of k_rows_adj that take a single TGV slice's v step off its
+  // wide row items: one pixel per thread.
+  static int vstep_groups(const SliceDev& d) { return (d.W * d.H + kRowAdjThreadsWide - 1) / kRowAdjThreadsWide; }
 
   void upload_lists(const std::vector<int2>& rows, const std::vector<int2>& rgs, const std::vector<int2>& rgs_a,
                     const std::vector<int2>& cols, const std::vector<GroupRange>& gr) {
@@ -969,6 +974,12 @@ struct Engine {
     // whose gap terms follow the row items'
     const int ndual = dense && sv[0]->is_op() ? dual_groups(hs[0]) : 0;
     if (one && sv[0]->is_op()) hs[0].rg_end = hs[0].rg_beg + (int)rgs.size() + ndual;
+    // and its v step, with wide adjoint blocks (the kernel falls back to the
+    // row items, writing zero gap terms, where the slice does not allow it)
+    const bool tgv = kind == PDT_SOLVER_TGV_MRI || kind == PDT_SOLVER_TGV_OP;
+    const int nvstep =
+        dense && sv[0]->is_op() && tgv && L.cb_a * hs[0].W >= 8 * kRowThreads ? vstep_groups(hs[0]) : 0;
+    if (one && sv[0]->is_op()) hs[0].row_end = hs[0].row_beg + (int)rows.size() + nvstep;
     HIPC(hipMemcpy(d_slices.p, hs.data(), hs.size() * sizeof(SliceDev), hipMemcpyHostToDevice));
     if (!rows.empty()) HIPC(hipMemcpy(d_rows.p, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
     if (!rgs.empty()) HIPC(hipMemcpy(d_rg.p, rgs.data(), rgs.size() * sizeof(int2), hipMemcpyHostToDevice));
@@ -1015,6 +1026,7 @@ struct Engine {
       }
       x.dense = dense;
       x.ndual = ndual;
+      x.nvstep = nvstep;
       listed_items += r.nrow + r.nrg + r.nrga + r.ncol;
       G.push_back(x);
     }

@@ -1331,12 +1331,35 @@ __device__ __forceinline__ void mri_primal_pixel(const SliceDev& d, int x, int y
 // would take more), so that two workgroups share a CU.
 template <class R, int MODE, bool GAP, int NT, int NF>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1024 && NF > 0 && ct_pad_ok(NF) ? 8 : 1))) void k_rows_adj(const SliceDev* sl, const int2* items, int nitems, int cb,
-                                                 double* part_rows) {
+                                                 double* part_rows, int nvstep) {
   extern __shared__ char smem_raw[];
+  using C = Cx<R>;
+  if (int(blockIdx.x) >= nitems) {
+    // nvstep workgroups after the row items (a single slice): the v step of
+    // TGV_primal_v_kernel on contiguous pixel ranges, where the row items
+    // would do it before their FFT (gap terms at part_rows[row_beg + H + b]);
+    // elsewhere they only write zero gap terms.
+    const SliceDev& d = sl[0];
+    if (!slice_active(d)) return;
+    const int b = int(blockIdx.x) - nitems, W = d.W, WH = W * d.H;
+    double t[2] = {0, 0};
+    if (MODE == M_TGV && NF > 0 && d.ngr_a == 1 && 2 * W <= NT) {
+      const int chunk = (WH + nvstep - 1) / nvstep;
+      const int pa = min(WH, b * chunk), pe = min(WH, pa + chunk);
+      const R tau = R(d.tau), theta = R(d.theta);
+      const float rW = 1.0f / float(W);
+      for (int i = pa + threadIdx.x; i < pe; i += NT) {
+        const int y = qdiv(i, W, rW), x = i - y * W;
+        const double rv = vstep_pixel<C, C>(d, x, y, tau, theta);
+        if (GAP) t[1] += rv;
+      }
+    }
+    if (GAP) block_sum<2, NT>(t, part_rows + size_t(d.row_beg + d.H + b) * kGapTerms);
+    return;
+  }
   const int li = xcd_item(blockIdx.x, nitems);
   const int2 it = items ? items[li] : make_int2(0, li);  // null: the dense single-slice list
   const SliceDev& d = sl[it.x];
-  using C = Cx<R>;
   const int W = NF > 0 ? NF : d.W, H = d.H, nc = d.ncols;
   const int y = it.y / d.ngr_a, g = it.y - y * d.ngr_a;
   const int c0 = g * cb, ncb = min(cb, d.C - c0);
@@ -1423,7 +1446,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1024 &
       const auto px = gptr<const C>(d.p);
       e_uo = gptr<const C>(d.u)[i];
       e_div = dmx<C>(px, x, i, W) + dmy<C>(px + n, y, i, W, H);
-    } else if (MODE == M_TGV && x < 2 * W) {
+    } else if (MODE == M_TGV && nvstep == 0 && x < 2 * W) {
       const double rv = vstep_pixel<C, C>(d, x - W, y, tau, theta);
       if (GAP) t[1] += rv;
     }
@@ -1810,11 +1833,12 @@ template <class R, int MODE, int NF>
 static void launch_rows_adj(const Launch& L, bool gap) {
   constexpr int NR = kRowThreads, NW = kRowAdjThreadsWide;
   if (L.adj_threads == NW) {
-    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NW, NF>), L.nrg_a, NW, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
-    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NW, NF>), L.nrg_a, NW, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
+    const int nv = L.dense ? L.nvstep : 0;
+    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NW, NF>), L.nrg_a + nv, NW, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, nv);
+    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NW, NF>), L.nrg_a + nv, NW, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, nv);
   } else {
-    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NR, NF>), L.nrg_a, NR, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
-    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NR, NF>), L.nrg_a, NR, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows);
+    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NR, NF>), L.nrg_a, NR, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, 0);
+    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NR, NF>), L.nrg_a, NR, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, 0);
   }
 }
 

@@ -45,6 +45,9 @@ struct Launch {
   // single dense slice: workgroups of k_rows_fwd that do the dual step alone
   // after the row items (0: the row items share it out)
   int ndual = 0;
+  // single dense slice, TGV, wide adjoint blocks: workgroups of k_rows_adj
+  // that do the v step alone after the row items (0: the row items do it)
+  int nvstep = 0;
   // workgroup size of the denoising half-iterations (256, 512 or 1024): the
   // smallest that covers rpi rows of the widest slice in one pass, at most 1024
   int den_threads = 256;
