@@ -1053,7 +1053,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GAP ? 5 : 1)
   if (MODE != M_OP && int(blockIdx.x) >= nitems) {
     const SliceDev& d = sl[0];
     if (!slice_active(d)) return;
-    const int b = int(blockIdx.x) - nitems, W = d.W, WH = W * d.H;
+    // consecutive pixel ranges on one XCD, so that the next row's pixels
+    // (forward differences) are mostly in the same L2
+    const int b = xcd_item(int(blockIdx.x) - nitems, ndual), W = d.W, WH = W * d.H;
     const int chunk = (WH + ndual - 1) / ndual;
     const int pa = min(WH, b * chunk), pe = min(WH, pa + chunk);
     const int k = gptr<int>(d.ctl)[CTL_ITER];
@@ -1341,7 +1343,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1024 &
     // elsewhere they only write zero gap terms.
     const SliceDev& d = sl[0];
     if (!slice_active(d)) return;
-    const int b = int(blockIdx.x) - nitems, W = d.W, WH = W * d.H;
+    // consecutive pixel ranges on one XCD (backward differences of q)
+    const int b = xcd_item(int(blockIdx.x) - nitems, nvstep), W = d.W, WH = W * d.H;
     double t[2] = {0, 0};
     if (MODE == M_TGV && NF > 0 && d.ngr_a == 1 && 2 * W <= NT) {
       const int chunk = (WH + nvstep - 1) / nvstep;
@@ -1584,7 +1587,7 @@ __global__ __launch_bounds__(1024) void k_mri_primal(const SliceDev* sl, const i
 // over the row items.  MRI: part[0] rows_fwd, part[2] rows_adj over the row
 // items, part[1] cols over the column items.
 // ---------------------------------------------------------------------------
-constexpr int kGapThreads = 1024;  // one workgroup per slice, loads of many items in flight
+constexpr int kGapThreads = kBlock;  // one workgroup per slice (1024 threads measured slower)
 
 template <class R>
 __global__ __launch_bounds__(kGapThreads) void k_gap_reduce(const SliceDev* sl, int kind, const double* p0,
