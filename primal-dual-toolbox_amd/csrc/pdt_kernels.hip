@@ -785,6 +785,15 @@ __device__ __forceinline__ double vstep_pixel(const SliceDev& d, int x, int y, t
 
 __device__ __forceinline__ bool slice_active(const SliceDev& d) { return gptr<const int>(d.ctl)[CTL_ACTIVE] != 0; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Give XCD x a contiguous range of logical work items, so that items
+// touching neighbouring memory (adjacent column groups, adjacent rows) share
+// one L2.  Bijective on [0, n).
+__device__ __forceinline__ int xcd_item(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
+  return x * q + min(x, r) + k;
+}
+
 // ---------------------------------------------------------------------------
 // Denoising half-iterations.  Item = (slice, first row); rpi rows per item.
 // ---------------------------------------------------------------------------
@@ -793,7 +802,8 @@ __device__ __forceinline__ bool slice_active(const SliceDev& d) { return gptr<co
 template <class P, bool GAP, int NT>
 __global__ __launch_bounds__(NT) void k_tv_dual(const SliceDev* sl, const int2* items, int rpi,
                                                     double* part) {
-  const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
+  const int bi = xcd_item(blockIdx.x, gridDim.x);  // neighbouring rows on one XCD
+  const int2 it = items ? items[bi] : make_int2(0, bi * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using R = typename RealOf<P>::T;
@@ -813,7 +823,7 @@ __global__ __launch_bounds__(NT) void k_tv_dual(const SliceDev* sl, const int2* 
       t[3] += dabs2(ub[i] - f[i]);
     }
   }
-  if (GAP) block_sum<4, NT>(reinterpret_cast<double(&)[4]>(t), part + size_t(blockIdx.x) * kGapTerms);
+  if (GAP) block_sum<4, NT>(reinterpret_cast<double(&)[4]>(t), part + size_t(bi) * kGapTerms);
 }
 
 // TV_primal_u_kernel 2D (tgv_kernels.cuh:568-605).  Gap: ROF dual terms
@@ -821,7 +831,8 @@ __global__ __launch_bounds__(NT) void k_tv_dual(const SliceDev* sl, const int2* 
 template <class P, bool GAP, int NT>
 __global__ __launch_bounds__(NT) void k_tv_primal(const SliceDev* sl, const int2* items, int rpi,
                                                       double* part) {
-  const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
+  const int bi = xcd_item(blockIdx.x, gridDim.x);  // neighbouring rows on one XCD
+  const int2 it = items ? items[bi] : make_int2(0, bi * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using R = typename RealOf<P>::T;
@@ -851,7 +862,7 @@ __global__ __launch_bounds__(NT) void k_tv_primal(const SliceDev* sl, const int2
       t[1] += dabs2(div);
     }
   }
-  if (GAP) block_sum<2, NT>(t, part + size_t(blockIdx.x) * kGapTerms);
+  if (GAP) block_sum<2, NT>(t, part + size_t(bi) * kGapTerms);
   if (ya == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += 1;
 }
 
@@ -859,7 +870,8 @@ __global__ __launch_bounds__(NT) void k_tv_primal(const SliceDev* sl, const int2
 // then TGV_primal_v_kernel at every pixel.
 template <class P, class S, int NT>
 __global__ __launch_bounds__(NT) void k_tgv_primal(const SliceDev* sl, const int2* items, int rpi) {
-  const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
+  const int bi = xcd_item(blockIdx.x, gridDim.x);  // neighbouring rows on one XCD
+  const int2 it = items ? items[bi] : make_int2(0, bi * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using R = typename RealOf<P>::T;
@@ -891,7 +903,8 @@ __global__ __launch_bounds__(NT) void k_tgv_primal(const SliceDev* sl, const int
 template <class P, class S, bool GAP, int NT>
 __global__ __launch_bounds__(NT) void k_tgv_dual(const SliceDev* sl, const int2* items, int rpi,
                                                      double* part) {
-  const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
+  const int bi = xcd_item(blockIdx.x, gridDim.x);  // neighbouring rows on one XCD
+  const int2 it = items ? items[bi] : make_int2(0, bi * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   using R = typename RealOf<P>::T;
@@ -911,7 +924,7 @@ __global__ __launch_bounds__(NT) void k_tgv_dual(const SliceDev* sl, const int2*
       t[3] += dabs2(ub[i] - f[i]);
     }
   }
-  if (GAP) block_sum<4, NT>(reinterpret_cast<double(&)[4]>(t), part + size_t(blockIdx.x) * kGapTerms);
+  if (GAP) block_sum<4, NT>(reinterpret_cast<double(&)[4]>(t), part + size_t(bi) * kGapTerms);
   if (ya == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += 1;
 }
 
@@ -919,7 +932,8 @@ __global__ __launch_bounds__(NT) void k_tgv_dual(const SliceDev* sl, const int2*
 template <class P, class S>
 __global__ __launch_bounds__(kBlock) void k_tgv_restore1(const SliceDev* sl, const int2* items, int rpi,
                                                          double* part) {
-  const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
+  const int bi = xcd_item(blockIdx.x, gridDim.x);  // neighbouring rows on one XCD
+  const int2 it = items ? items[bi] : make_int2(0, bi * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   const int W = d.W, H = d.H;
@@ -943,14 +957,15 @@ __global__ __launch_bounds__(kBlock) void k_tgv_restore1(const SliceDev* sl, con
     m = fmax(m, sqrt(dabs2(a) + dabs2(b)));
   }
   m = block_max(m);
-  if (threadIdx.x == 0) part[size_t(blockIdx.x) * kGapTerms + 0] = m;
+  if (threadIdx.x == 0) part[size_t(bi) * kGapTerms + 0] = m;
 }
 
 // Step 2: g = div d, Re<g, f>, ||g||^2.
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_tgv_restore2(const SliceDev* sl, const int2* items, int rpi,
                                                          double* part) {
-  const int2 it = items ? items[blockIdx.x] : make_int2(0, int(blockIdx.x) * rpi);
+  const int bi = xcd_item(blockIdx.x, gridDim.x);  // neighbouring rows on one XCD
+  const int2 it = items ? items[bi] : make_int2(0, bi * rpi);
   const SliceDev& d = sl[it.x];
   if (!slice_active(d)) return;
   const int W = d.W, H = d.H;
@@ -969,7 +984,7 @@ __global__ __launch_bounds__(kBlock) void k_tgv_restore2(const SliceDev* sl, con
     t[0] += redot(g, f[i]);
     t[1] += dabs2(g);
   }
-  block_sum<2>(t, part + size_t(blockIdx.x) * kGapTerms);
+  block_sum<2>(t, part + size_t(bi) * kGapTerms);
 }
 
 // ---------------------------------------------------------------------------
@@ -978,14 +993,6 @@ __global__ __launch_bounds__(kBlock) void k_tgv_restore2(const SliceDev* sl, con
 // ---------------------------------------------------------------------------
 enum { M_OP = 0, M_TV = 1, M_TGV = 2 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-// L2).  Give XCD x a contiguous range of logical work items, so that items
-// touching neighbouring memory (adjacent column groups, adjacent rows) share
-// one L2.  Bijective on [0, n).
-__device__ __forceinline__ int xcd_item(int b, int n) {
-  const int q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
-  return x * q + min(x, r) + k;
-}
 
 // LDS layout of the row FFT passes: twiddles (W), two FFT buffers (cb * W
 // each), listed columns (nc ints).
