@@ -195,6 +195,37 @@ def _batch_pair(pdt, specs):
     return out
 
 
+def test_single_slice_step_workgroups_equal_batch():
+    """A single-slice solve runs the dual step and, with wide adjoint blocks,
+    the v step in workgroups of their own; a batch runs both inside the row
+    workgroups.  320 x 320, 8 coils, the same tiles (all coils of a row in one
+    adjoint block) for both: the iterates agree bit for bit after 100
+    iterations, the gap to 1e-12."""
+    pdt = _pdt()
+    from primaldualtoolbox import synthetic as syn
+    case = syn.mri_case(seed=31, n=320, coils=8)
+    a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
+    single = _mri_solver(pdt, case, a1=a1, max_iter=99, gap_every=50)[1]
+    batch = [_mri_solver(pdt, case, a1=a1, max_iter=99, gap_every=50)[1] for _ in range(2)]
+    pdt.mri.set_tiles([single], cb=4, cg=1, cb_a=8)
+    pdt.mri.set_tiles(batch, cb=4, cg=1, cb_a=8)
+    try:
+        single.solve()
+        pdt.mri.batch_solve(batch)
+    finally:
+        pdt.mri.set_tiles([single])
+        pdt.mri.set_tiles(batch)
+    s0 = single.getStats()
+    for b in batch:
+        sb = b.getStats()
+        print(f"  single {s0['iters']} it. gap {s0['rel_gap']:.12e}; batch {sb['iters']} it. gap {sb['rel_gap']:.12e}",
+              flush=True)
+        assert sb["iters"] == s0["iters"] == 100
+        assert abs(sb["rel_gap"] - s0["rel_gap"]) <= 1e-12 * abs(s0["rel_gap"])
+        assert np.array_equal(b.getResult(), single.getResult())
+        assert np.array_equal(b.getV(), single.getV())
+
+
 def test_batch_ragged_stops_equal_single_solves():
     """Slices of one batch that stop at different iterations (iteration
     limits and gap tolerances) leave the work lists as they finish; every
