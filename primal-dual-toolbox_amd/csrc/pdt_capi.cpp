@@ -278,6 +278,11 @@ struct pdt_op {
   int nrows = 0, nrg = 0, ncolitems = 0, cb = 1, cg = 1;
   size_t lds_rows = 0, lds_cols = 0;
   hipStream_t stream = nullptr;
+  // host source of the descriptor copy (pinned, outlives the asynchronous
+  // copy) and the completion of the last application, which shares the
+  // descriptor and the scratch planes with the next one
+  SliceDev* h_desc = nullptr;
+  hipEvent_t done = nullptr;
 
   int required() const { return kind == PDT_OP_SAMPLING ? 1 : 2; }
   const char* name() const {
@@ -465,7 +470,11 @@ struct pdt_op {
     d.op_in = src;
     d.op_out = dst;
     d.ctl = ctl.as<int>();
-    HIPC(hipMemcpyAsync(slice.p, &d, sizeof(d), hipMemcpyHostToDevice, st));
+    if (!h_desc) HIPC(hipHostMalloc((void**)&h_desc, sizeof(SliceDev), hipHostMallocDefault));
+    if (done) HIPC(hipEventSynchronize(done));  // the previous application (any stream) has finished
+    else HIPC(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    *h_desc = d;
+    HIPC(hipMemcpyAsync(slice.p, h_desc, sizeof(d), hipMemcpyHostToDevice, st));
     Launch L;
     L.stream = st;
     L.sl = slice.as<SliceDev>();
@@ -498,9 +507,15 @@ struct pdt_op {
       else run_zero(st, dst, size_t(W) * H * esize(cplx_of(dtype)));
     }
     HIPC(hipGetLastError());
+    HIPC(hipEventRecord(done, st));
   }
 
   ~pdt_op() {
+    if (done) {
+      (void)hipEventSynchronize(done);
+      (void)hipEventDestroy(done);
+    }
+    if (h_desc) (void)hipHostFree(h_desc);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -573,18 +588,29 @@ void default_params(int kind, pdt_params* p) {
   (void)kind;
 }
 
+// State buffers of a W x H solve.  They are kept while the shape is unchanged:
+// a new input (setInput0) then overwrites them in place, so the cached
+// engine's descriptors and captured graphs, which hold these addresses, stay
+// valid, and no memory that a graph refers to is freed and reallocated.
 void alloc_state(pdt_solver* s, int W, int H) {
   const size_t n = size_t(W) * H, e = s->pe();
-  const int64_t us[2] = {W, H};
-  s->u.reset(new_array(s->dtype, 2, us));
-  s->ub.alloc(n * e);
-  s->p.alloc(2 * n * s->pq_e());
+  auto exact = [](DevBuf& b, size_t bytes) {
+    if (b.bytes != bytes) b.alloc(bytes);
+  };
+  if (!s->u || s->u->size[0] != W || s->u->size[1] != H) {
+    const int64_t us[2] = {W, H};
+    s->u.reset(new_array(s->dtype, 2, us));
+  }
+  exact(s->ub, n * e);
+  exact(s->p, 2 * n * s->pq_e());
   if (s->is_tgv()) {
-    const int64_t vs[3] = {W, H, 2};
-    s->v.reset(new_array(s->dtype, 3, vs));
-    s->vb.alloc(2 * n * e);
-    s->q.alloc(3 * n * s->pq_e());
-    if (s->kind == PDT_SOLVER_TGV) s->tmp.alloc(2 * n * e);
+    if (!s->v || s->v->size[0] != W || s->v->size[1] != H) {
+      const int64_t vs[3] = {W, H, 2};
+      s->v.reset(new_array(s->dtype, 3, vs));
+    }
+    exact(s->vb, 2 * n * e);
+    exact(s->q, 3 * n * s->pq_e());
+    if (s->kind == PDT_SOLVER_TGV) exact(s->tmp, 2 * n * e);
   }
   s->W = W;
   s->H = H;
@@ -974,11 +1000,16 @@ struct Engine {
     // whose gap terms follow the row items'
     const int ndual = dense && sv[0]->is_op() ? dual_groups(hs[0]) : 0;
     if (one && sv[0]->is_op()) hs[0].rg_end = hs[0].rg_beg + (int)rgs.size() + ndual;
-    // and its v step, with wide adjoint blocks (the kernel falls back to the
-    // row items, writing zero gap terms, where the slice does not allow it)
+    // and its v step, where the wide adjoint blocks would otherwise do it
+    // before their FFT: the same conditions as the kernel's (1024-thread
+    // adjoint blocks, a compile-time FFT length, two threads per pixel of a
+    // row, one adjoint coil group)
     const bool tgv = kind == PDT_SOLVER_TGV_MRI || kind == PDT_SOLVER_TGV_OP;
-    const int nvstep =
-        dense && sv[0]->is_op() && tgv && L.cb_a * hs[0].W >= 8 * kRowThreads ? vstep_groups(hs[0]) : 0;
+    const int nvstep = dense && sv[0]->is_op() && tgv && L.cb_a * hs[0].W >= 8 * kRowThreads &&
+                               fft_len_compiled(dtk(), hs[0].W) && 2 * hs[0].W <= kRowAdjThreadsWide &&
+                               hs[0].ngr_a == 1
+                           ? vstep_groups(hs[0])
+                           : 0;
     if (one && sv[0]->is_op()) hs[0].row_end = hs[0].row_beg + (int)rows.size() + nvstep;
     HIPC(hipMemcpy(d_slices.p, hs.data(), hs.size() * sizeof(SliceDev), hipMemcpyHostToDevice));
     if (!rows.empty()) HIPC(hipMemcpy(d_rows.p, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
@@ -1316,14 +1347,31 @@ void prepare_solve(pdt_solver* s, int verbose) {
   if (s->is_tgv() && s->kind == PDT_SOLVER_TGV && !s->tmp.p) s->tmp.alloc(2 * size_t(s->W) * s->H * s->pe());
 }
 
-void solve_many(std::vector<pdt_solver*>& sv, int verbose) {
-  for (auto* s : sv) prepare_solve(s, verbose);
-  for (size_t i = 1; i < sv.size(); ++i) {
-    if (sv[i]->kind != sv[0]->kind || sv[i]->dtype != sv[0]->dtype)
-      fail(PDT_ERR_ARG, "batched solvers must share kind and dtype");
-    if (sv[i]->prm.gap_every != sv[0]->prm.gap_every)
-      fail(PDT_ERR_ARG, "batched solvers must share gap_every");
+// What one engine (one set of launches) requires of the solvers it runs
+// together: distinct handles, one kernel family (kind, precision), one gap
+// schedule, and one storage size of the dual fields p, q.  The engine
+// instantiates its kernels from the first solver, so a solver whose p, q were
+// sized otherwise would be written out of bounds.  Checked on the host before
+// any state is touched or any kernel is launched.
+void check_batch(const std::vector<pdt_solver*>& sv) {
+  if (sv.empty()) fail(PDT_ERR_ARG, "empty batch");
+  for (size_t i = 0; i < sv.size(); ++i) {
+    if (!sv[i]) fail(PDT_ERR_ARG, "null solver in batch");
+    for (size_t j = 0; j < i; ++j)
+      if (sv[j] == sv[i]) fail(PDT_ERR_ARG, "a solver appears twice in the batch");
   }
+  const pdt_solver* s0 = sv[0];
+  for (size_t i = 1; i < sv.size(); ++i) {
+    const pdt_solver* s = sv[i];
+    if (s->kind != s0->kind || s->dtype != s0->dtype) fail(PDT_ERR_ARG, "batched solvers must share kind and dtype");
+    if (s->prm.gap_every != s0->prm.gap_every) fail(PDT_ERR_ARG, "batched solvers must share gap_every");
+    if (s->pq_e() != s0->pq_e()) fail(PDT_ERR_ARG, "batched solvers must share dual_fp16");
+  }
+}
+
+void solve_many(std::vector<pdt_solver*>& sv, int verbose) {
+  check_batch(sv);
+  for (auto* s : sv) prepare_solve(s, verbose);
   if (sv.size() == 1) {
     // reuse the engine of the previous solve when nothing it depends on
     // changed: its work lists and instantiated graphs stay valid
@@ -1644,9 +1692,11 @@ int pdt_solver_get_params(const pdt_solver* s, pdt_params* p) {
 int pdt_solver_set_params(pdt_solver* s, const pdt_params* p) {
   return guard([&] {
     if (!s || !p) fail(PDT_ERR_ARG, "null argument");
+    // validated before anything is assigned: a refused call changes nothing
+    // (p and q keep the storage size they were allocated with)
     const bool fp16_changed = (p->dual_fp16 != 0) != (s->prm.dual_fp16 != 0);
-    s->prm = *p;
     if (fp16_changed && s->have_u) fail(PDT_ERR_STATE, "dual_fp16 must be set before the first solve");
+    s->prm = *p;
   });
 }
 
@@ -1826,6 +1876,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* ou
   return guard([&] {
     if (!solvers || count <= 0 || reps <= 0) fail(PDT_ERR_ARG, "bad argument");
     std::vector<pdt_solver*> sv(solvers, solvers + count);
+    check_batch(sv);
     for (auto* s : sv) prepare_solve(s, 0);
     const bool mri = sv[0]->is_op();
     // candidates: coils per forward row item, columns per column item, rows
@@ -1879,6 +1930,7 @@ int pdt_set_tiles(pdt_solver** solvers, int count, int cb, int cg, int rpi, int 
   return guard([&] {
     if (!solvers || count <= 0) fail(PDT_ERR_ARG, "bad argument");
     std::vector<pdt_solver*> sv(solvers, solvers + count);
+    check_batch(sv);
     for (auto* s : sv) prepare_solve(s, 0);
     Engine e;
     e.sv = sv;
@@ -1898,8 +1950,6 @@ int pdt_batch_solve(pdt_solver** solvers, int count, int verbose) {
   return guard([&] {
     if (!solvers || count <= 0) fail(PDT_ERR_ARG, "empty batch");
     std::vector<pdt_solver*> v(solvers, solvers + count);
-    for (auto* s : v)
-      if (!s) fail(PDT_ERR_ARG, "null solver in batch");
     solve_many(v, verbose);
   });
 }

@@ -1837,6 +1837,13 @@ static void with_fft_len(int n, Fn&& fn) {
   fn(std::integral_constant<int, 0>{});
 }
 
+bool fft_len_compiled(int dtype, int n) {
+  if (dtype != DT_C64 && dtype != DT_F32) return false;
+  for (int m : kFftLengths)
+    if (m == n) return true;
+  return false;
+}
+
 // Adjoint row pass with L.adj_threads threads per block (kRowThreads, or
 // kRowAdjThreadsWide for items holding many coils).
 template <class R, int MODE, int NF>

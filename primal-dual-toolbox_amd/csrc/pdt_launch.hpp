@@ -78,6 +78,10 @@ void run_sum_strided(hipStream_t st, const double* a, int n, int stride, double*
 // out[0] = sum of |z_i|^2 over n complex pixels (DT_C64 or DT_C128).
 void run_sumsq(hipStream_t st, int dtype, const void* z, int64_t n, double* out);
 
+// True when the row/column kernels of this precision have a compile-time
+// FFT plan for length n (the kernels fall back to the runtime plan otherwise).
+bool fft_len_compiled(int dtype, int n);
+
 // Allows the MRI kernels the full LDS of a workgroup (call before launches);
 // returns the dynamic LDS bytes a workgroup may use.
 size_t init_kernel_attrs();

@@ -22,7 +22,7 @@ import time
 import numpy as np
 
 from . import _capi as C
-from . import mri
+from . import pymrireconstruction as mri
 from . import synthetic as syn
 from . import warmstart as ws
 

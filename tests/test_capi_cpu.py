@@ -149,3 +149,80 @@ def test_committed_warm_start_weights_match_the_network():
     assert (depth, width) == (5, 32)
     assert p.dtype == np.float32 and p.size == ws.num_params(ws.layer_channels(depth, width))
     assert np.isfinite(p).all()
+
+
+def test_reference_module_names():
+    """The reference package imports the binding modules pymrireconstruction
+    and pydenoising and exposes them as mri and denoising
+    (primaldualtoolbox/__init__.py:21-22)."""
+    import importlib
+    import primaldualtoolbox as pdt
+    m = importlib.import_module("primaldualtoolbox.pymrireconstruction")
+    d = importlib.import_module("primaldualtoolbox.pydenoising")
+    assert pdt.mri is m and pdt.denoising is d
+    assert m.TgvMriOptimizer_2c3c is pdt.mri.TgvMriOptimizer_2c3c
+    assert d.TgvOptimizer_2c is pdt.denoising.TgvOptimizer_2c
+
+
+def _solver(kind, dtype, fp16=0):
+    from primaldualtoolbox import _capi as C
+    L = C.lib()
+    h = ct.c_void_p()
+    C.check(L.pdt_solver_create(kind, dtype, ct.byref(h)))
+    p = C.PdtParams()
+    C.check(L.pdt_params_default(kind, ct.byref(p)))
+    p.dual_fp16 = fp16
+    C.check(L.pdt_solver_set_params(h, ct.byref(p)))
+    return h
+
+
+def test_batch_checks_fire_before_any_launch():
+    """Solvers batched into one engine must share kind, dtype, gap_every and
+    the storage size of p, q; a handle may appear once.  The checks run on
+    the host before any state is touched (no GPU needed here), for the
+    batched solve, the autotuner and the manual tile choice alike."""
+    from primaldualtoolbox import _capi as C
+    L = C.lib()
+    a = _solver(C.PDT_SOLVER_TGV, C.PDT_C64, 0)
+    b = _solver(C.PDT_SOLVER_TGV, C.PDT_C64, 1)  # fp16 p, q: half the bytes
+    c = _solver(C.PDT_SOLVER_TV, C.PDT_C64)
+    d = _solver(C.PDT_SOLVER_TGV, C.PDT_C64, 0)
+    try:
+        arr = lambda *hs: (ct.c_void_p * len(hs))(*hs)  # noqa: E731
+        out = [ct.c_int() for _ in range(4)]
+        ms = ct.c_double()
+        cases = [(arr(a, b), "share dual_fp16"), (arr(a, c), "share kind and dtype"),
+                 (arr(a, d, a), "appears twice")]
+        for hs, msg in cases:
+            assert L.pdt_batch_solve(hs, len(hs), 0) == C.PDT_ERR_ARG
+            assert msg.encode() in L.pdt_last_error()
+            assert L.pdt_autotune(hs, len(hs), 1, *[ct.byref(o) for o in out], ct.byref(ms)) == C.PDT_ERR_ARG
+            assert msg.encode() in L.pdt_last_error()
+            assert L.pdt_set_tiles(hs, len(hs), 1, 1, 1, 1) == C.PDT_ERR_ARG
+            assert msg.encode() in L.pdt_last_error()
+        # a consistent batch gets past the checks to the reference's state error
+        assert L.pdt_batch_solve(arr(a, d), 2, 0) == C.PDT_ERR_STATE
+        assert b"No noisy data is set!" in L.pdt_last_error()
+    finally:
+        for h in (a, b, c, d):
+            L.pdt_solver_destroy(h)
+
+
+def test_params_before_any_state_are_free_to_change():
+    """Before the first setInput0/solve no state exists, so dual_fp16 may
+    still change (the GPU tests check the refusal once p, q exist)."""
+    from primaldualtoolbox import _capi as C
+    L = C.lib()
+    h = _solver(C.PDT_SOLVER_TGV, C.PDT_C64, 1)
+    try:
+        p = C.PdtParams()
+        C.check(L.pdt_solver_get_params(h, ct.byref(p)))
+        p.dual_fp16, p.max_iter = 0, 77
+        assert L.pdt_solver_set_params(h, ct.byref(p)) == C.PDT_OK  # no state yet: allowed
+        p.dual_fp16 = 1
+        assert L.pdt_solver_set_params(h, ct.byref(p)) == C.PDT_OK
+        q = C.PdtParams()
+        C.check(L.pdt_solver_get_params(h, ct.byref(q)))
+        assert (q.dual_fp16, q.max_iter) == (1, 77)
+    finally:
+        L.pdt_solver_destroy(h)
