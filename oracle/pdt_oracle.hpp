@@ -256,6 +256,10 @@ struct FftPlan {
   int n = 0;
   std::vector<int> radix;
   std::vector<cx<R>> tw;  // exp(-2 pi i m / n), m in [0, n), evaluated in long double
+  // per stage s: stage twiddles w^(j k n / (p r)) for k < p, 1 <= j < r, at
+  // stw[s][k * (r - 1) + j - 1] (forward sign); and the radix-r roots
+  // exp(-2 pi i m / r) at root[s][m]
+  std::vector<std::vector<cx<R>>> stw, root;
 };
 
 inline std::vector<int> factorize(int n) {
@@ -268,53 +272,128 @@ inline std::vector<int> factorize(int n) {
 
 template <class R>
 const FftPlan<R>& fft_plan(int n) {
+  // one plan per length, built once; the thread-local map makes the lookup
+  // lock-free after the first call of a thread
+  thread_local std::map<int, const FftPlan<R>*> local;
+  auto hit = local.find(n);
+  if (hit != local.end()) return *hit->second;
   static std::mutex mu;
   static std::map<int, std::unique_ptr<FftPlan<R>>> cache;
   std::lock_guard<std::mutex> lk(mu);
   auto it = cache.find(n);
-  if (it != cache.end()) return *it->second;
-  auto pl = std::make_unique<FftPlan<R>>();
-  pl->n = n;
-  pl->radix = factorize(n);
-  pl->tw.resize(n);
-  const long double twopi = 6.283185307179586476925286766559L;
-  for (int m = 0; m < n; ++m) {
-    long double a = -twopi * (long double)m / (long double)n;
-    pl->tw[m] = cx<R>(R(std::cos(a)), R(std::sin(a)));
+  if (it == cache.end()) {
+    auto pl = std::make_unique<FftPlan<R>>();
+    pl->n = n;
+    pl->radix = factorize(n);
+    pl->tw.resize(n);
+    const long double twopi = 6.283185307179586476925286766559L;
+    for (int m = 0; m < n; ++m) {
+      long double a = -twopi * (long double)m / (long double)n;
+      pl->tw[m] = cx<R>(R(std::cos(a)), R(std::sin(a)));
+    }
+    int p = 1;
+    for (int r : pl->radix) {
+      const int nr = n / r, ts = n / (p * r);
+      std::vector<cx<R>> st(size_t(p) * (r - 1)), rt(r);
+      for (int k = 0; k < p; ++k)
+        for (int j = 1; j < r; ++j) st[size_t(k) * (r - 1) + j - 1] = pl->tw[size_t((long long)j * k * ts % n)];
+      for (int m = 0; m < r; ++m) rt[m] = pl->tw[size_t(m) * nr];
+      pl->stw.push_back(std::move(st));
+      pl->root.push_back(std::move(rt));
+      p *= r;
+    }
+    it = cache.emplace(n, std::move(pl)).first;
   }
-  auto& ref = *pl;
-  cache[n] = std::move(pl);
-  return ref;
+  local[n] = it->second.get();
+  return *it->second;
+}
+
+// Radix-r DFT of x (in place), sign -1 forward / +1 inverse; root[m] =
+// exp(-2 pi i m / r).  Radices 2-5 with their closed forms, others direct.
+template <class R>
+inline void small_dft(cx<R>* x, int r, int sign, const cx<R>* root) {
+  const R s = R(sign);
+  auto rot = [&](cx<R> v) { return cx<R>(-s * v.imag(), s * v.real()); };  // v * (s i)
+  switch (r) {
+    case 2: {
+      const cx<R> a = x[0], b = x[1];
+      x[0] = a + b;
+      x[1] = a - b;
+      return;
+    }
+    case 3: {
+      const R h = R(0.86602540378443864676372317075294L);  // sin(2 pi / 3)
+      const cx<R> t = x[1] + x[2], d = rot(x[1] - x[2]) * h, m = x[0] - t * R(0.5);
+      x[0] = x[0] + t;
+      x[1] = m + d;
+      x[2] = m - d;
+      return;
+    }
+    case 4: {
+      const cx<R> t0 = x[0] + x[2], t1 = x[0] - x[2], t2 = x[1] + x[3], t3 = rot(x[1] - x[3]);
+      x[0] = t0 + t2;
+      x[2] = t0 - t2;
+      x[1] = t1 + t3;
+      x[3] = t1 - t3;
+      return;
+    }
+    case 5: {
+      const R c1 = R(0.30901699437494742410229341718281906L), c2 = R(-0.80901699437494742410229341718281906L);
+      const R s1 = R(0.95105651629515357211643933337938214L), s2 = R(0.58778525229247312916870595463907277L);
+      const cx<R> t1 = x[1] + x[4], t2 = x[2] + x[3], d1 = x[1] - x[4], d2 = x[2] - x[3];
+      const cx<R> a1 = x[0] + t1 * c1 + t2 * c2, a2 = x[0] + t1 * c2 + t2 * c1;
+      const cx<R> b1 = rot(d1 * s1 + d2 * s2), b2 = rot(d1 * s2 - d2 * s1);
+      x[0] = x[0] + t1 + t2;
+      x[1] = a1 + b1;
+      x[4] = a1 - b1;
+      x[2] = a2 + b2;
+      x[3] = a2 - b2;
+      return;
+    }
+    default: {
+      cx<R> y[64];
+      for (int q = 0; q < r; ++q) {
+        cx<R> acc = x[0];
+        int m = 0;
+        for (int j = 1; j < r; ++j) {
+          m += q;
+          if (m >= r) m -= r;
+          const cx<R> w = sign > 0 ? std::conj(root[m]) : root[m];
+          acc += x[j] * w;
+        }
+        y[q] = acc;
+      }
+      for (int q = 0; q < r; ++q) x[q] = y[q];
+    }
+  }
 }
 
 // In place on a (contiguous, length n); work must hold n elements.
+// Stockham autosort: stage radix r, p = product of the earlier radices,
+// nr = n / r; for i < nr, k = i mod p: x_j = in[i + j nr] w^(j k n/(p r)),
+// y = DFT_r(x), out[(i - k) r + k + q p] = y_q.
 template <class R>
 void fft1(cx<R>* a, int n, int sign, cx<R>* work) {
   if (n <= 1) return;
   const FftPlan<R>& pl = fft_plan<R>(n);
-  auto tw = [&](long long m) {
-    cx<R> w = pl.tw[size_t(m % n)];
-    return sign > 0 ? std::conj(w) : w;
-  };
   cx<R>* in = a;
   cx<R>* out = work;
   int p = 1;
-  cx<R> xin[64];
-  for (int r : pl.radix) {
-    const int nr = n / r;
-    for (int i = 0; i < nr; ++i) {
-      const int k = i % p;
-      for (int j = 0; j < r; ++j) {
-        cx<R> v = in[i + j * nr];
-        xin[j] = (j && k) ? v * tw((long long)j * k * (n / (p * r))) : v;
+  cx<R> x[64];
+  for (size_t s = 0; s < pl.radix.size(); ++s) {
+    const int r = pl.radix[s], nr = n / r;
+    const cx<R>* st = pl.stw[s].data();
+    const cx<R>* rt = pl.root[s].data();
+    for (int i0 = 0; i0 < nr; i0 += p)  // i = i0 + k, k < p (p divides nr)
+      for (int k = 0; k < p; ++k) {
+        const int i = i0 + k;
+        x[0] = in[i];
+        const cx<R>* w = st + size_t(k) * (r - 1);
+        for (int j = 1; j < r; ++j) x[j] = in[i + j * nr] * (sign > 0 ? std::conj(w[j - 1]) : w[j - 1]);
+        small_dft(x, r, sign, rt);
+        cx<R>* o = out + size_t(i0) * r + k;
+        for (int q = 0; q < r; ++q) o[q * p] = x[q];
       }
-      const int base = (i - k) * r + k;
-      for (int q = 0; q < r; ++q) {
-        cx<R> s = 0;
-        for (int j = 0; j < r; ++j) s += xin[j] * tw((long long)((j * q) % r) * nr);
-        out[base + q * p] = s;
-      }
-    }
     std::swap(in, out);
     p *= r;
   }
