@@ -129,6 +129,21 @@ PDT_API int pdt_op_num_constants(const pdt_op* op, int* have, int* required);
  * pre-allocated at the right size; sizes and constants are checked first. */
 PDT_API int pdt_op_forward(pdt_op* op, const pdt_array* src, pdt_array* dst);
 PDT_API int pdt_op_adjoint(pdt_op* op, const pdt_array* src, pdt_array* dst);
+/* NEW, no counterpart in the reference: pdt_op_forward (forward != 0) or
+ * pdt_op_adjoint (forward == 0) without any device allocation, for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  src and dst as for those calls.
+ * Enqueues on `stream` (a hipStream_t; NULL = the null stream) and returns
+ * without waiting.  The intermediates live in one array of
+ * PDT_NOALLOC_SCRATCH_BYTES per GPU, part of the library's code object and
+ * held until the process ends: every call on a GPU shares it, so calls that
+ * overlap on one GPU race; each call rewrites everything it reads.
+ * PDT_ERR_STATE until one pdt_op_forward, pdt_op_adjoint or
+ * pdt_op_adjointness has prepared the operator (that call allocates);
+ * PDT_ERR_SIZE when the intermediates (2 C H nc + ceil(C / 4) H W complex
+ * values for C coils, H x W images and nc sampled lines) exceed the array. */
+#define PDT_NOALLOC_SCRATCH_BYTES (32u << 20)
+PDT_API int pdt_op_apply_noalloc(pdt_op* op, int forward, const pdt_array* src, pdt_array* dst, void* stream);
 /* getInputSize / getOutputSize (mricartesianoperator.cu:183-194,296-308). */
 PDT_API int pdt_op_input_size(pdt_op* op, int out_ndim, const int64_t* out_size,
                               int* in_ndim, int64_t* in_size);

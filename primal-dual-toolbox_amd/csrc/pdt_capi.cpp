@@ -510,6 +510,80 @@ struct pdt_op {
     HIPC(hipEventRecord(done, st));
   }
 
+  // apply() without any device allocation: the intermediates (descriptor,
+  // control words, T1, T2, per-group partial adjoints) live in the per-GPU
+  // scratch array, the work lists are the identity (null lists), and the
+  // descriptor is written by a kernel from its by-value argument.  Needs the
+  // prepared geometry (prepare() allocates) and enqueues on st without
+  // waiting.  Calls that overlap on one GPU share the array and race.
+  void apply_noalloc(bool forward, const void* src, void* dst, hipStream_t st) {
+    int cbn = 1, cgn = 1;
+    size_t lr = 0, lc = 0;
+    choose_tiles(cbn, cgn, lr, lc);
+    const size_t ce = esize(cplx_of(dtype));
+    const int ngr = (C + cbn - 1) / cbn, ng = (ncols + cgn - 1) / cgn;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t o_ctl = al(sizeof(SliceDev));
+    const size_t o_t1 = o_ctl + al(CTL_WORDS * sizeof(int));
+    const size_t tb = al(size_t(C) * H * ncols * ce);
+    const size_t o_t2 = o_t1 + tb, o_wp = o_t2 + tb;
+    const size_t need = o_wp + (ngr > 1 ? al(size_t(ngr) * H * W * ce) : 0);
+    if (need > kNoAllocScratchBytes) {
+      std::ostringstream ss;
+      ss << prefix() << "the allocation-free application needs " << need << " bytes of scratch, above its maximum of "
+         << kNoAllocScratchBytes;
+      fail(PDT_ERR_SIZE, ss.str());
+    }
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    char* base = static_cast<char*>(noalloc_scratch(dev));
+    if (!base) fail(PDT_ERR_HIP, "the scratch array of the allocation-free application is not available on this device");
+    SliceDev d = geometry();
+    d.ngroups = ng;
+    d.ngr = ngr;
+    d.ngr_a = ngr;
+    d.T1 = base + o_t1;
+    d.T2 = base + o_t2;
+    d.wpart = ngr > 1 ? base + o_wp : nullptr;
+    d.op_in = src;
+    d.op_out = dst;
+    d.ctl = reinterpret_cast<int*>(base + o_ctl);
+    SliceDev* d_dev = reinterpret_cast<SliceDev*>(base);
+    run_store_desc(st, d, d_dev, d.ctl);
+    Launch L;
+    L.stream = st;
+    L.sl = d_dev;
+    L.nsl = 1;
+    L.rows = nullptr;  // identity lists: rows y, (row, coil group), (coil, column group)
+    L.nrows = H;
+    L.rowgroups = nullptr;
+    L.nrg = H * ngr;
+    L.cols = nullptr;
+    L.ncolitems = C * ng;
+    L.cb = cbn;
+    L.cg = cgn;
+    L.lds_rows = lr;
+    L.lds_cols = lc;
+    L.fft_w = W;
+    L.fft_h = Hk;
+    L.split_primal = ngr > 1;
+    L.primal_threads = primal_block(W);
+    L.rowgroups_a = nullptr;
+    L.nrg_a = L.nrg;
+    L.cb_a = cbn;
+    L.lds_rows_a = lr;
+    L.adj_threads = kRowThreads;
+    const int dt = dtype == PDT_C128 ? DT_C128 : DT_C64;
+    if (forward) {
+      run_zero(st, dst, size_t(W) * Hk * C * ce);
+      if (ncols) run_op_forward(L, dt);
+    } else {
+      if (ncols) run_op_adjoint(L, dt);
+      else run_zero(st, dst, size_t(W) * H * ce);
+    }
+    HIPC(hipGetLastError());
+  }
+
   ~pdt_op() {
     if (done) {
       (void)hipEventSynchronize(done);
@@ -1549,6 +1623,19 @@ int pdt_op_adjoint(pdt_op* op, const pdt_array* src, pdt_array* dst) {
     op->prepare();
     op->apply(false, src->ptr, dst->ptr, nullptr);
     HIPC(hipStreamSynchronize(nullptr));
+  });
+}
+
+int pdt_op_apply_noalloc(pdt_op* op, int forward, const pdt_array* src, pdt_array* dst, void* stream) {
+  return guard([&] {
+    if (!op || !src || !dst) fail(PDT_ERR_ARG, "null argument");
+    if (forward) op_size_check(op, src, dst);
+    else op_size_check(op, dst, src);
+    if (!op->prepared)
+      fail(PDT_ERR_STATE, op->prefix() +
+                              "not prepared: the first pdt_op_forward, pdt_op_adjoint or pdt_op_adjointness "
+                              "prepares its geometry");
+    op->apply_noalloc(forward != 0, src->ptr, dst->ptr, static_cast<hipStream_t>(stream));
   });
 }
 

@@ -28,6 +28,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <type_traits>
 
 #include "pdt_desc.hpp"
@@ -1560,7 +1561,7 @@ template <class R, int MODE, bool GAP>
 __global__ __launch_bounds__(1024) void k_mri_primal(const SliceDev* sl, const int2* items, int nitems, int rpi,
                                                      double* part) {
   const int li = xcd_item(blockIdx.x, nitems);
-  const int2 it = items[li];
+  const int2 it = items ? items[li] : make_int2(0, li * rpi);  // null: the dense single-slice list
   const SliceDev& d = sl[it.x];
   if (!slice_active(d) || d.ngr_a == 1) return;
   using C = Cx<R>;
@@ -1745,6 +1746,14 @@ __global__ __launch_bounds__(kBlock) void k_compact(const SliceDev* sl, const in
     }
   }
   block_sum<1>(t, part + size_t(blockIdx.x) * kGapTerms);
+}
+
+// Writes a slice descriptor passed by value to device memory and opens its
+// control words (active, iteration 0): the set-up of an operator application
+// that copies nothing from the host.
+__global__ __launch_bounds__(64) void k_store_desc(SliceDev d, SliceDev* dst, int* ctl) {
+  if (threadIdx.x == 0) *dst = d;
+  if (threadIdx.x < CTL_WORDS) ctl[threadIdx.x] = threadIdx.x == CTL_ACTIVE ? 1 : 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_zero(uint32_t* p, size_t n) {
@@ -2076,6 +2085,33 @@ void run_sum_strided(hipStream_t st, const double* a, int n, int stride, double*
 void run_sumsq(hipStream_t st, int dtype, const void* z, int64_t n, double* out) {
   if (dtype == DT_C128) PDT_LAUNCH_NT((k_sumsq<double>), 1, 1024, 0, st, static_cast<const Cx<double>*>(z), n, out);
   else PDT_LAUNCH_NT((k_sumsq<float>), 1, 1024, 0, st, static_cast<const Cx<float>*>(z), n, out);
+}
+
+void run_store_desc(hipStream_t st, const SliceDev& d, SliceDev* dst, int* ctl) {
+  hipLaunchKernelGGL(k_store_desc, dim3(1), dim3(64), 0, st, d, dst, ctl);
+}
+
+// Scratch of the allocation-free operator application: one array per GPU,
+// part of the code object (no allocation), held until the process ends.
+// Host code never passes the array's name to a kernel or a copy (on the host
+// it is a stand-in, not a device address); it uses the per-device address
+// that hipGetSymbolAddress returns, taken once per device with that device
+// current.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+void* noalloc_scratch(int device) {
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[device]) {
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != device) return nullptr;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    addr[device] = p;
+  }
+  return addr[device];
 }
 
 void run_zero(hipStream_t st, void* ptr, size_t bytes) {

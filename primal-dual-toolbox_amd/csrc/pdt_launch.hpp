@@ -71,6 +71,18 @@ void run_op_adjoint(const Launch& L, int dtype);
 // zeroes r, and writes per-block partial sums of |f|^2 over columns without
 // samples into f0_part (nblocks entries, returned).
 int run_compact_data(const Launch& L, int dtype, const void* f_full, double* f0_part);
+// Writes slice descriptor d to dst and opens its control words ctl (a
+// kernel: nothing is copied from the host).
+void run_store_desc(hipStream_t st, const SliceDev& d, SliceDev* dst, int* ctl);
+
+// Bytes of the per-GPU scratch array of the allocation-free operator
+// application, and the largest device index it serves.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr int kMaxDevices = 64;
+// Device address of that array on `device`, which must be the current
+// device (looked up once per device); null on failure.
+void* noalloc_scratch(int device);
+
 // Fills n elements of the given size (bytes per element 4, 8 or 16) with 0.
 void run_zero(hipStream_t st, void* ptr, size_t bytes);
 // out[0] = sum of a[i * stride], i < n (one block, fixed order).

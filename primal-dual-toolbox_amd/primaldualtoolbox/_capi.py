@@ -25,7 +25,7 @@ EXPORTS = [
     "pdt_array_info", "pdt_array_device_ptr", "pdt_array_destroy",
     "pdt_op_create", "pdt_op_add_constant", "pdt_op_num_constants", "pdt_op_forward",
     "pdt_op_adjoint", "pdt_op_input_size", "pdt_op_output_size", "pdt_op_adjointness",
-    "pdt_op_name", "pdt_op_destroy",
+    "pdt_op_name", "pdt_op_destroy", "pdt_op_apply_noalloc",
     "pdt_solver_create", "pdt_params_default", "pdt_solver_get_params", "pdt_solver_set_params",
     "pdt_solver_set_operator", "pdt_solver_set_noisy_data", "pdt_solver_set_input0",
     "pdt_solver_solve", "pdt_solver_get_result", "pdt_solver_get_v", "pdt_solver_get_stats",
@@ -88,6 +88,7 @@ def lib():
         L.pdt_op_num_constants.argtypes = [vp, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]
         L.pdt_op_forward.argtypes = [vp, vp, vp]
         L.pdt_op_adjoint.argtypes = [vp, vp, vp]
+        L.pdt_op_apply_noalloc.argtypes = [vp, ct.c_int, vp, vp, vp]
         L.pdt_op_input_size.argtypes = [vp, ct.c_int, i64p, ct.POINTER(ct.c_int), i64p]
         L.pdt_op_output_size.argtypes = [vp, ct.c_int, i64p, ct.POINTER(ct.c_int), i64p]
         dp = ct.POINTER(ct.c_double)
