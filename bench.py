@@ -65,22 +65,65 @@ def cold_start(opt, u0_dev):
     C.check(C.lib().pdt_solver_set_input0(opt._h, u0_dev.h, None))
 
 
+def host_info() -> dict:
+    """The host the CPU baseline runs on: logical CPUs, physical cores and
+    model (lscpu), and the OpenMP threads the oracle gets."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+        cores = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        info["physical_cores"] = len({ln for ln in cores.splitlines() if ln and not ln.startswith("#")})
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def default_cpu_threads() -> int:
+    """OMP_NUM_THREADS where the host sets it (the CPU share of this GPU on
+    a shared machine), else every physical core."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return int(host_info().get("physical_cores") or os.cpu_count() or 1)
+
+
+def timed_cpu(run, runs: int = 5) -> tuple[float, list]:
+    """Median seconds of `runs` timed calls after one untimed warm-up."""
+    run()
+    ts = [run() for _ in range(runs)]
+    return float(np.median(ts)), ts
+
+
 def cpu_baseline(case, a1, gpu_iters: int, sample_iters: int, threads: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.set_threads(threads)
     op = O.Op(O.OP_CARTESIAN_ROOS, case["sens"], case["mask"], prec=32)
-    _, _, st, _ = O.solve(O.TGV_MRI, case["kspace"], op=op, alpha1=a1, alpha0=2 * a1,
-                          max_iter=sample_iters - 1, prec=32)
-    per_iter = st["seconds"] / sample_iters
+
+    def run():
+        _, _, st, _ = O.solve(O.TGV_MRI, case["kspace"], op=op, alpha1=a1, alpha0=2 * a1,
+                              max_iter=sample_iters - 1, prec=32)
+        return st["seconds"]
+    med, ts = timed_cpu(run)
+    per_iter = med / sample_iters
+    host = host_info()
+    host["omp_max_threads"] = O.max_threads()
     return {
         "value": per_iter * gpu_iters,
         "unit": "s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{sample_iters} TGV-MRI iterations of the same 320x320x15 workload in the FP32 "
-                   f"OpenMP oracle ({per_iter * 1e3:.1f} ms/iteration), scaled to the GPU's "
+        "sample": (f"FP32 OpenMP oracle on {threads} threads, the same 320x320x15 workload: median of 5 timed "
+                   f"runs of {sample_iters} iterations after 1 warm-up ({per_iter * 1e3:.2f} ms/iteration, "
+                   f"timed with steady_clock around the iteration loop), extrapolated to the GPU's "
                    f"{gpu_iters} iterations to rel. gap {GAP_TOL:g}"),
+        "method": "extrapolated",
+        "runs_s": ts,
+        "host": host,
     }
 
 
@@ -198,9 +241,11 @@ def extra_tv256(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
     if not no_cpu:
         O = _oracle()
         O.set_threads(cpu_threads)
-        _, _, ost, _ = O.solve(O.TV, noisy, lambda_=10, max_iter=499, gap_every=500, prec=32)
-        out["cpu_baseline"] = {"value": ost["seconds"], "unit": "s", "cores": cpu_threads, "kind": "port",
-                               "sample": "the full 500 iterations in the FP32 OpenMP oracle"}
+        med, ts = timed_cpu(lambda: O.solve(O.TV, noisy, lambda_=10, max_iter=499, gap_every=500,
+                                            prec=32)[2]["seconds"])
+        out["cpu_baseline"] = {"value": med, "unit": "s", "cores": cpu_threads, "kind": "port",
+                               "sample": "the full 500 iterations in the FP32 OpenMP oracle, median of 5 after 1 warm-up",
+                               "runs_s": ts}
     return out
 
 
@@ -233,11 +278,12 @@ def extra_tgv512(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
         O = _oracle()
         O.set_threads(cpu_threads)
         n_it = 100
-        _, _, ost, _ = O.solve(O.TGV, noisy, lambda_=20, alpha0=2, alpha1=1, max_iter=n_it - 1,
-                               gap_every=n_it, prec=32)
-        out["cpu_baseline"] = {"value": ost["seconds"] * 1000 / n_it, "unit": "s", "cores": cpu_threads,
-                               "kind": "port",
-                               "sample": f"{n_it} iterations in the FP32 OpenMP oracle, scaled to 1000"}
+        med, ts = timed_cpu(lambda: O.solve(O.TGV, noisy, lambda_=20, alpha0=2, alpha1=1, max_iter=n_it - 1,
+                                            gap_every=n_it, prec=32)[2]["seconds"])
+        out["cpu_baseline"] = {"value": med * 1000 / n_it, "unit": "s", "cores": cpu_threads,
+                               "kind": "port", "method": "extrapolated",
+                               "sample": (f"{n_it} iterations in the FP32 OpenMP oracle (median of 5 after 1 "
+                                          f"warm-up), scaled to 1000"), "runs_s": ts}
     return out
 
 
@@ -277,19 +323,23 @@ def extra_demo384(steps: int, cpu_threads: int, no_cpu: bool, autotune: bool = T
         O.set_threads(cpu_threads)
         n_it = 20
         oop = O.Op(O.OP_CARTESIAN_ROOS, case["sens"], case["mask"], prec=32)
-        _, _, ost, _ = O.solve(O.TGV_MRI, case["kspace"], op=oop, alpha1=a1, alpha0=2 * a1, max_iter=n_it - 1,
-                               prec=32)
-        out["cpu_baseline"] = {"value": ost["seconds"] * it / n_it, "unit": "s", "cores": cpu_threads,
-                               "kind": "port",
-                               "sample": f"{n_it} iterations in the FP32 OpenMP oracle, scaled to {it}"}
+        med, ts = timed_cpu(lambda: O.solve(O.TGV_MRI, case["kspace"], op=oop, alpha1=a1, alpha0=2 * a1,
+                                            max_iter=n_it - 1, prec=32)[2]["seconds"])
+        out["cpu_baseline"] = {"value": med * it / n_it, "unit": "s", "cores": cpu_threads,
+                               "kind": "port", "method": "extrapolated",
+                               "sample": (f"{n_it} iterations in the FP32 OpenMP oracle (median of 5 after 1 "
+                                          f"warm-up), scaled to {it}"), "runs_s": ts}
     return out
 
 
 def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool, autotune: bool = True,
-                  graph: bool = True) -> dict:
+                  graph: bool = True, runs: int = 3) -> dict:
     """`count` mixed Cartesian TGV-MRI slices (sizes 256^2..384x320, 8-20
     coils, random or regular masks) solved together to rel. gap 1e-4 in one
-    batched solve; reports slices/s and per-slice iterations."""
+    batched solve (median of `runs` timed solves); slices/s, per-slice
+    time-to-gap (alone, and to its stop inside the batch), algorithmic bytes
+    per batch iteration and the per-kernel roofline of the batched launches
+    (the working set, ~3 GB, is far beyond the 256 MB Infinity Cache)."""
     import primaldualtoolbox as pdt
     from primaldualtoolbox import _capi as C
     from primaldualtoolbox import synthetic as syn
@@ -327,30 +377,56 @@ def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool, auto
 
     cold()
     pdt.mri.batch_solve(solvers)  # warmup
-    cold()
-    t0 = time.perf_counter()
-    pdt.mri.batch_solve(solvers)
-    batch_s = time.perf_counter() - t0
+    times = []
+    for r in range(runs):
+        cold()
+        t0 = time.perf_counter()
+        pdt.mri.batch_solve(solvers)
+        times.append(time.perf_counter() - t0)
+        log(f"[bench] batch run {r}: {times[-1]:.3f} s")
+    batch_s = float(np.median(times))
     stats = [o.getStats() for o in solvers]
     its = [int(s["iters"]) for s in stats]
+    stop_ms = [float(s["ms_to_stop"]) for s in stats]
     if not all(s["converged"] for s in stats):
         raise RuntimeError("batched solve: a slice did not reach the gap")
-    log(f"[bench] batch: {count} slices in {batch_s:.3f} s, iterations {min(its)}..{max(its)}")
+    # the launches of one batch iteration with every slice active
+    cold()
+    prof = pdt.mri.batch_profile(solvers, reps=10)
+    per_iter_ms = sum(p["ms"] for p in prof)
+    it_bytes = sum(p["bytes"] for p in prof)
+    dom = max(prof, key=lambda p: p["ms"])
     # the same slices one at a time (single-slice engine per solver)
     cold()
     t0 = time.perf_counter()
+    alone_ms = []
     for o in solvers:
         o.solve()
+        alone_ms.append(float(o.getStats()["ms"]))
     seq_s = time.perf_counter() - t0
     log(f"[bench] batch: sequential {seq_s:.3f} s")
-    work = [int(s["iters"]) * c["meta"]["coils"] * 2 * c["meta"]["n"] * c["meta"]["n_pe"]
-            for s, c in zip(stats, cases)]
+    work = [it * c["meta"]["coils"] * 2 * c["meta"]["n"] * c["meta"]["n_pe"] for it, c in zip(its, cases)]
+
+    def dist(x):
+        return {"min": float(np.min(x)), "median": float(np.median(x)), "max": float(np.max(x))}
     out = {"workload": (f"{count} Cartesian TGV-MRI slices (seeds 1000..{999 + count}), sizes "
                         f"{sorted(set((c['meta']['n'], c['meta']['n_pe']) for c in cases))}, 8-20 coils, "
                         "rel. gap 1e-4 per slice, cold start"),
-           "seconds": batch_s, "slices_per_s": count / batch_s,
+           "seconds": batch_s, "seconds_runs": times, "slices_per_s": count / batch_s,
            "sequential_seconds": seq_s, "sequential_slices_per_s": count / seq_s,
-           "iterations": {"min": min(its), "median": int(np.median(its)), "max": max(its)},
+           "iterations": {k: int(v) for k, v in dist(its).items()},
+           "time_to_gap_alone_ms": dist(alone_ms),
+           "time_to_stop_in_batch_ms": dist(stop_ms),
+           "per_slice": [{"seed": c["meta"]["seed"], "n": c["meta"]["n"], "n_pe": c["meta"]["n_pe"],
+                          "coils": c["meta"]["coils"], "iterations": it, "alone_ms": round(a, 3),
+                          "in_batch_stop_ms": round(b, 1)}
+                         for c, it, a, b in zip(cases, its, alone_ms, stop_ms)],
+           "batch_iteration": {"ms": per_iter_ms, "alg_MB": it_bytes / 1e6,
+                               "GBs": it_bytes / (per_iter_ms * 1e-3) / 1e9,
+                               "kernels": _kernels(prof)},
+           "roofline": {"kernel": dom["name"], "bound": "hbm",
+                        "achieved": dom["bytes"] / (dom["ms"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": dom["bytes"] / (dom["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
            "tiles": tune}
     if not no_cpu:
         O = _oracle()
@@ -370,10 +446,12 @@ def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool, auto
         per_unit = sum(t for t, _ in sample) / sum(u for _, u in sample)
         cpu_s = per_unit * sum(work) / n_s
         out["cpu_baseline"] = {"value": count / cpu_s, "unit": "slices/s", "cores": n_s, "kind": "port",
+                               "method": "extrapolated",
                                "sample": (f"{n_it} iterations of each of the first {n_s} slices in the FP32 "
                                           f"oracle, one slice per thread; cost per coil-pixel-iteration "
                                           f"scaled to the {count} slices' GPU iteration counts and divided "
-                                          f"over {n_s} threads")}
+                                          f"over {n_s} threads"),
+                               "host": host_info()}
     return out
 
 
@@ -410,7 +488,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-iters", type=int, default=40)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads of the CPU baseline (0: OMP_NUM_THREADS if set, else all physical cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=50)
     ap.add_argument("--fixed-iters", type=int, default=0,
@@ -431,6 +510,8 @@ def main():
                     help="comma-separated side measurements to run alone (cnn, tv_256, tgv_512, demo_384, "
                          "batch), printed as one JSON line {\"extra\": ...} instead of the headline")
     args = ap.parse_args()
+    if args.cpu_threads <= 0:
+        args.cpu_threads = default_cpu_threads()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(slices would shard across GPUs without communication; only the one-GPU share is built)",
@@ -507,6 +588,13 @@ def main():
     u_cold = opt.getResult().copy()
     warm = None if args.no_warm else warm_start_steps(opt, case, args.steps, args.warmup)
     value = float(np.mean(warm["seconds"])) if warm else cold_s
+    # the FP64 oracle's solution of the same slice at its own stop (gap 1e-4)
+    oracle_end = None
+    gold = os.path.join(ROOT, "tests", "golden", "endstate_320x320x15.npz")
+    if os.path.exists(gold):
+        with np.load(gold) as z:
+            oracle_end = {"iterations": int(z["iters_0"]), "rel_l2_cold": _rel(u_cold, z["u_0"]),
+                          "rel_l2_warm": _rel(warm["u"], z["u_0"]) if warm else None}
 
     # per-kernel durations (hipEvents on the solver stream) and algorithmic bytes
     prof = opt.profileKernels(args.profile_reps)
@@ -556,6 +644,7 @@ def main():
             "warm_network_ms": float(np.median(warm["network_ms"])) if warm else None,
             "warm_vs_cold_rel_l2_u": _rel(warm["u"], u_cold) if warm else None,
             "iterations_to_gap": int(np.median(iters)),
+            "fp64_oracle_end_state": oracle_end,
             "final_rel_gap": float(np.median(gaps)),
             "device_loop_ms": float(np.median(loop_ms)),
             "us_per_iteration": float(np.median(loop_ms)) * 1e3 / max(1, int(np.median(iters))),
@@ -571,6 +660,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": ("static: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel, committed in "
+                               "profiles/traffic_tgv_mri_320x320x15.json (not measured in this run)"
+                               if traffic is not None else None),
         },
         "cpu_baseline": cpu,
         "extra": extra,

@@ -96,6 +96,9 @@ typedef struct pdt_stats {
   double rel_gap;     /* (P - D) / |P|                                         */
   double ms;          /* device time of the iteration loop (hipEvent), ms      */
   int32_t converged;  /* 1 if gap_tol was reached                              */
+  double ms_to_stop;  /* NEW: in a batched solve, host time from the start of the
+                         loop to the end of the chunk in which this slice stopped;
+                         otherwise equal to ms                                  */
 } pdt_stats;
 
 /* ---- library ------------------------------------------------------------ */
@@ -202,22 +205,33 @@ typedef struct pdt_kernel_prof {
   double bytes;
 } pdt_kernel_prof;
 PDT_API int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, int cap, int* n);
+/* NEW: pdt_solver_profile over the launches of a batch (pdt_batch_solve):
+ * per kernel, the duration of one launch over every slice and the
+ * algorithmic bytes of that launch summed over the slices. */
+PDT_API int pdt_batch_profile(pdt_solver** solvers, int count, int reps, pdt_kernel_prof* out, int cap, int* n);
 
 /* NEW: autotuner.  Times every candidate tile shape (coils per LDS chunk of
  * the forward and of the adjoint row pass, columns per group of the column
  * pass, rows per work item of the denoising passes) on the configured solvers
- * (one batch) with hipEvents, keeps the
- * fastest in the process-wide tile cache (persisted to the file named by the
- * environment variable PDT_TUNE_CACHE, if set) and returns it.  Advances the
- * solver state: call before the solves and reset with set_input0. */
+ * (one batch) with hipEvents; then, with those tiles, caps of 1, 2, 3, 4 or
+ * 6 workgroups per CU (3, 4 or 6 for denoising) on each pass in turn, each
+ * pass timed alone, keeping a cap only where it beats no cap by 2 %.  Keeps
+ * the choice in the process-wide tile cache (persisted to the file named by
+ * the environment variable PDT_TUNE_CACHE, if set) and returns it: wpc
+ * (4 ints, may be NULL) receives the caps of the forward row, column,
+ * adjoint row and denoising passes (0: none); ms_per_iteration the stage
+ * times of the choice.  Advances the solver state: call before the solves
+ * and reset with set_input0. */
 PDT_API int pdt_autotune(pdt_solver** solvers, int count, int reps, int* cb, int* cg, int* rpi, int* cb_a,
-                         double* ms_per_iteration);
+                         int* wpc, double* ms_per_iteration);
 
 /* NEW: a manual tile choice for the shape of these solvers (one batch),
  * stored in the same tile cache as pdt_autotune's result; values <= 0 take
- * the default, values above a shape's limit are clamped.  The next solve of
- * the same shape uses them. */
-PDT_API int pdt_set_tiles(pdt_solver** solvers, int count, int cb, int cg, int rpi, int cb_a);
+ * the default, values above a shape's limit are clamped.  wpc (4 ints or
+ * NULL = no caps): workgroups per CU of the forward row, column, adjoint row
+ * and denoising passes, 0 = no cap, at most 6.  The next solve of the same
+ * shape uses them. */
+PDT_API int pdt_set_tiles(pdt_solver** solvers, int count, int cb, int cg, int rpi, int cb_a, const int* wpc);
 
 /* NEW: solve many configured solvers (same kind and dtype) together, with one
  * persistent work-list launch per half-iteration for the whole batch.  Each

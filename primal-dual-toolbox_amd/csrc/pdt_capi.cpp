@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -783,6 +784,7 @@ struct Engine {
   int kind = 0, dtype = 0;
   bool fp16 = false;
   int force_cb = 0, force_cg = 0, force_rpi = 0, force_cb_a = 0;  // autotuner overrides (0: cache / default)
+  int force_wpc[4] = {-1, -1, -1, -1};  // occupancy caps (-1: cache; 0: none)
   bool plan_only = false;                         // build(): stop after the signature
   std::string sig;
   TuneKey key{};
@@ -850,6 +852,7 @@ struct Engine {
       }
       key = TuneKey{kind, dt, maxW, sv[0]->H, maxHk, maxC, maxNc, ns};
       TuneVal tv = tune_lookup(key);
+      for (int i = 0; i < 4; ++i) L.wpc[i] = force_wpc[i] >= 0 ? force_wpc[i] : tv.wpc[i];
       cb = force_cb > 0 ? force_cb : tv.cb > 0 ? tv.cb : kDefaultCb;
       cg = force_cg > 0 ? force_cg : tv.cg > 0 ? tv.cg : kDefaultCg;
       cb_a = force_cb_a > 0 ? force_cb_a : tv.cb_a > 0 ? tv.cb_a : cb;
@@ -862,6 +865,7 @@ struct Engine {
     } else {
       key = TuneKey{kind, dt, maxW, sv[0]->H, 0, 0, 0, ns};
       TuneVal tv = tune_lookup(key);
+      for (int i = 0; i < 4; ++i) L.wpc[i] = force_wpc[i] >= 0 ? force_wpc[i] : tv.wpc[i];
       rpi = force_rpi > 0 ? force_rpi : tv.rpi > 0 ? tv.rpi : 1;
     }
     // slices, grouped by FFT lengths, and their work lists
@@ -960,7 +964,7 @@ struct Engine {
     // signature of everything the kernels and graphs depend on (the control
     // and statistics words are owned by the engine and added below)
     sig.assign(reinterpret_cast<const char*>(hs.data()), hs.size() * sizeof(SliceDev));
-    const int tiles[5] = {cb, cg, rpi, int(sv[0]->prm.gap_every), cb_a};
+    const int tiles[9] = {cb, cg, rpi, int(sv[0]->prm.gap_every), cb_a, L.wpc[0], L.wpc[1], L.wpc[2], L.wpc[3]};
     sig.append(reinterpret_cast<const char*>(tiles), sizeof(tiles));
     if (plan_only) return;
     cs_bytes = size_t(ns) * (ST_WORDS * sizeof(double) + CTL_WORDS * sizeof(int));
@@ -1266,6 +1270,7 @@ struct Engine {
     constexpr unsigned kChunk = 50;
     const unsigned chunk = std::min<unsigned>(gap ? gap_every : total, kChunk);
     const bool use_graph = sv[0]->prm.use_graph != 0;
+    std::vector<double> stop_ms(ns, -1.0);  // batched solves: time from the start to each slice's stop
     reset_ctl();
     hipEvent_t e0, e1;
     HIPC(hipEventCreate(&e0));
@@ -1337,12 +1342,18 @@ struct Engine {
       }
     } else {
       unsigned k = 0;
+      const auto t0 = std::chrono::steady_clock::now();
       while (k < total) {
         const unsigned b = boundary(k);
         launch_chunk(int(b - k), gap && (b % gap_every) == 0, use_graph);
         copy_ctl(h_cs);
         HIPC(hipStreamSynchronize(stream));
         k = b;
+        // the end of the chunk in which a slice stopped (at its gap, seen in
+        // the copied control words, or at its limit, below) is its time to stop
+        const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (int i = 0; i < ns; ++i)
+          if (stop_ms[i] < 0 && (!h_ctl[size_t(i) * CTL_WORDS + CTL_ACTIVE] || limit[i] <= k)) stop_ms[i] = el;
         // a slice stops at its own iteration limit (host-driven, so that no
         // kernel ever reads a control word that another block of the same
         // launch writes)
@@ -1379,6 +1390,7 @@ struct Engine {
       s->stats.gap = st[ST_GAP];
       s->stats.rel_gap = st[ST_REL];
       s->stats.ms = ms;
+      s->stats.ms_to_stop = stop_ms[i] >= 0 ? stop_ms[i] : ms;
       s->stats.converged = c[CTL_DONE_AT] >= 0;
       s->rupd = c[CTL_RUPD];
     }
@@ -1863,12 +1875,16 @@ int pdt_solver_destroy(pdt_solver* s) {
   return guard([&] { delete s; });
 }
 
-int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, int cap, int* n) {
-  return guard([&] {
-    if (!s || !out || !n || reps <= 0) fail(PDT_ERR_ARG, "bad argument");
-    prepare_solve(s, 0);
+namespace {
+// Per-stage duration and algorithmic bytes of one launch, over a solver or a
+// batch (pdt_solver_profile, pdt_batch_profile).
+void profile_engine(std::vector<pdt_solver*> sv, int reps, pdt_kernel_prof* out, int cap, int* n) {
+  if (!out || !n || reps <= 0) fail(PDT_ERR_ARG, "bad argument");
+  check_batch(sv);
+  for (auto* s : sv) prepare_solve(s, 0);
+  {
     Engine e;
-    e.sv = {s};
+    e.sv = sv;
     e.build();
     e.reset_ctl();
     // The iteration as a solve runs it: a hipGraph of reps iterations, timed
@@ -1930,17 +1946,32 @@ int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, int cap, i
       ++k;
     }
     *n = k;
+  }
+}
+}  // namespace
+
+int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, int cap, int* n) {
+  return guard([&] {
+    if (!s) fail(PDT_ERR_ARG, "null solver");
+    profile_engine({s}, reps, out, cap, n);
+  });
+}
+
+int pdt_batch_profile(pdt_solver** solvers, int count, int reps, pdt_kernel_prof* out, int cap, int* n) {
+  return guard([&] {
+    if (!solvers || count <= 0) fail(PDT_ERR_ARG, "empty batch");
+    profile_engine(std::vector<pdt_solver*>(solvers, solvers + count), reps, out, cap, n);
   });
 }
 
 namespace {
-// Device time of one iteration (sum of the stage kernels), each stage timed
-// over `reps` back-to-back launches.
-double time_iteration(Engine& e, int reps) {
+// Device time of each stage kernel of an iteration (ms per launch), each
+// stage timed over `reps` back-to-back launches after one untimed launch.
+std::vector<double> time_stages(Engine& e, int reps) {
   hipEvent_t e0, e1;
   HIPC(hipEventCreate(&e0));
   HIPC(hipEventCreate(&e1));
-  double tot = 0;
+  std::vector<double> out;
   for (int st = 0; st < num_stages(e.kind); ++st) {
     e.stage(st, false);
     HIPC(hipEventRecord(e0, e.stream));
@@ -1950,25 +1981,31 @@ double time_iteration(Engine& e, int reps) {
     HIPC(hipGetLastError());
     float ms = 0;
     HIPC(hipEventElapsedTime(&ms, e0, e1));
-    tot += double(ms) / reps;
+    out.push_back(double(ms) / reps);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  return out;
+}
+double time_iteration(Engine& e, int reps) {
+  double tot = 0;
+  for (double t : time_stages(e, reps)) tot += t;
   return tot;
 }
 }  // namespace
 
 int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* out_cg, int* out_rpi, int* out_cb_a,
-                 double* out_ms) {
+                 int* out_wpc, double* out_ms) {
   return guard([&] {
     if (!solvers || count <= 0 || reps <= 0) fail(PDT_ERR_ARG, "bad argument");
     std::vector<pdt_solver*> sv(solvers, solvers + count);
     check_batch(sv);
     for (auto* s : sv) prepare_solve(s, 0);
     const bool mri = sv[0]->is_op();
-    // candidates: coils per forward row item, columns per column item, rows
-    // per denoising item, coils per adjoint row item (the forward choice, or
-    // every coil of the row, which also fuses the u step into that pass)
+    // Phase 1, tile shapes without occupancy caps: coils per forward row
+    // item, columns per column item, rows per denoising item, coils per
+    // adjoint row item (the forward choice, or every coil of the row, which
+    // also fuses the u step into that pass)
     std::vector<std::array<int, 4>> cand;
     if (mri) {
       int maxC = 1;
@@ -1985,35 +2022,82 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* ou
     } else {
       for (int rpi : {1, 2, 4, 8}) cand.push_back({0, 0, rpi, 0});
     }
+    auto engine = [&](const std::array<int, 4>& c, const int (&wpc)[4]) {
+      auto e = std::make_unique<Engine>();
+      e->sv = sv;
+      e->force_cb = c[0];
+      e->force_cg = c[1];
+      e->force_rpi = c[2];
+      e->force_cb_a = c[3];
+      for (int i = 0; i < 4; ++i) e->force_wpc[i] = wpc[i];
+      e->build();
+      e->reset_ctl();
+      return e;
+    };
+    const int none[4] = {0, 0, 0, 0};
     double best = 1e300;
     std::array<int, 4> bc{0, 0, 0, 0};
+    std::vector<double> best_st;
     TuneKey key{};
     for (const auto& c : cand) {
-      Engine e;
-      e.sv = sv;
-      e.force_cb = c[0];
-      e.force_cg = c[1];
-      e.force_rpi = c[2];
-      e.force_cb_a = c[3];
-      e.build();
-      e.reset_ctl();
-      const double ms = time_iteration(e, reps);
-      key = e.key;
+      auto e = engine(c, none);
+      const std::vector<double> st = time_stages(*e, reps);
+      double ms = 0;
+      for (double t : st) ms += t;
+      key = e->key;
       if (ms < best) {
         best = ms;
-        bc = {e.L.cb, e.L.cg, e.L.rpi, e.L.cb_a};
+        bc = {e->L.cb, e->L.cg, e->L.rpi, e->L.cb_a};
+        best_st = st;
       }
     }
-    tune_store(key, mri ? TuneVal{bc[0], bc[1], 0, bc[3]} : TuneVal{0, 0, bc[2], 0});
+    // Phase 2, waves per CU: with the chosen tiles, a cap on the workgroups
+    // per CU of each pass in turn (its kernel timed alone); a cap is kept
+    // only when it beats no cap by 2 %, so timing noise keeps the default.
+    // The denoising stages share one cap (their total is compared).
+    int wpc[4] = {0, 0, 0, 0};
+    std::vector<std::pair<int, std::vector<int>>> axes;  // (cap slot, stages it governs)
+    if (mri) axes = {{WPC_FWD, {0}}, {WPC_COLS, {1}}, {WPC_ADJ, {2}}};
+    else axes = {{WPC_DEN, {0, 1}}};
+    const std::vector<int> caps = mri ? std::vector<int>{1, 2, 3, 4, 6} : std::vector<int>{3, 4, 6};
+    for (const auto& ax : axes) {
+      auto cost = [&](const std::vector<double>& st) {
+        double t = 0;
+        for (int i : ax.second) t += st[size_t(i)];
+        return t;
+      };
+      double base = cost(best_st);
+      for (int cap : caps) {
+        int w[4];
+        for (int i = 0; i < 4; ++i) w[i] = wpc[i];
+        w[ax.first] = cap;
+        auto e = engine(bc, w);
+        const double t = cost(time_stages(*e, reps));
+        if (t < 0.98 * base) {
+          base = t;
+          wpc[ax.first] = cap;
+        }
+      }
+    }
+    TuneVal tv = mri ? TuneVal{bc[0], bc[1], 0, bc[3]} : TuneVal{0, 0, bc[2], 0};
+    for (int i = 0; i < 4; ++i) tv.wpc[i] = wpc[i];
+    tune_store(key, tv);
+    double ms = 0;
+    {
+      auto e = engine(bc, wpc);
+      ms = time_iteration(*e, reps);
+    }
     if (out_cb) *out_cb = bc[0];
     if (out_cg) *out_cg = bc[1];
     if (out_rpi) *out_rpi = bc[2];
     if (out_cb_a) *out_cb_a = bc[3];
-    if (out_ms) *out_ms = best;
+    if (out_wpc)
+      for (int i = 0; i < 4; ++i) out_wpc[i] = wpc[i];
+    if (out_ms) *out_ms = ms;
   });
 }
 
-int pdt_set_tiles(pdt_solver** solvers, int count, int cb, int cg, int rpi, int cb_a) {
+int pdt_set_tiles(pdt_solver** solvers, int count, int cb, int cg, int rpi, int cb_a, const int* wpc) {
   return guard([&] {
     if (!solvers || count <= 0) fail(PDT_ERR_ARG, "bad argument");
     std::vector<pdt_solver*> sv(solvers, solvers + count);
@@ -2027,9 +2111,12 @@ int pdt_set_tiles(pdt_solver** solvers, int count, int cb, int cg, int rpi, int 
     e.force_cg = cg > 0 ? cg : kDefaultCg;
     e.force_rpi = rpi > 0 ? rpi : 1;
     e.force_cb_a = cb_a > 0 ? cb_a : e.force_cb;
+    for (int i = 0; i < 4; ++i) e.force_wpc[i] = wpc ? std::max(0, std::min(wpc[i], 6)) : 0;
     e.build();
     const bool mri = sv[0]->is_op();
-    tune_store(e.key, mri ? TuneVal{e.L.cb, e.L.cg, 0, e.L.cb_a} : TuneVal{0, 0, e.L.rpi, 0});
+    TuneVal tv = mri ? TuneVal{e.L.cb, e.L.cg, 0, e.L.cb_a} : TuneVal{0, 0, e.L.rpi, 0};
+    for (int i = 0; i < 4; ++i) tv.wpc[i] = e.L.wpc[i];
+    tune_store(e.key, tv);
   });
 }
 

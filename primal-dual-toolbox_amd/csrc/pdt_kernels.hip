@@ -1765,6 +1765,20 @@ __global__ __launch_bounds__(kBlock) void k_zero(uint32_t* p, size_t n) {
 // ---------------------------------------------------------------------------
 #define PDT_LAUNCH(kern, grid, lds, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, __VA_ARGS__)
 
+// LDS of a CU on gfx950, its allocation granule, and an upper bound on the
+// static LDS of any kernel here (block reductions).
+constexpr size_t kLdsPerCu = 160 * 1024, kLdsGranule = 512, kStaticLdsMax = 2048;
+
+size_t lds_occupancy(size_t need, int wpc, size_t limit) {
+  if (wpc <= 0) return need;
+  // wpc workgroups of `share` bytes fit, wpc + 1 do not (wpc <= 6, so the
+  // static part's slack cannot let a further one in)
+  const size_t share = (kLdsPerCu / size_t(wpc)) / kLdsGranule * kLdsGranule;
+  if (share <= kStaticLdsMax) return need;
+  const size_t want = share - kStaticLdsMax;
+  return want > limit ? need : (want > need ? want : need);
+}
+
 size_t lds_rows_bytes(int dtype, int cb, int W, int nc, bool padded) {
   const size_t e = (dtype == DT_C128 || dtype == DT_F64) ? 16 : 8;
   return (size_t(W) + 2 * size_t(padded ? lds_pad(cb * W) : cb * W)) * e + size_t(nc) * sizeof(int);
@@ -1791,23 +1805,26 @@ static void denoise_stage(const Launch& L, int kind, bool gap, int stage) {
   const int n = L.nrows;
   if (n == 0) return;
   const int2* rows = L.dense ? nullptr : L.rows;
+  // occupancy cap: LDS only reserved (the kernels use none dynamically),
+  // within the 64 KiB a launch may ask for without a raised attribute
+  const size_t lds = lds_occupancy(0, L.wpc[WPC_DEN], 64 * 1024);
   // the half-iteration kernels with L.den_threads threads per workgroup
   auto launch = [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
     if (kind == K_TV) {
       if (stage == 0) {
-        if (gap) hipLaunchKernelGGL((k_tv_dual<P, true, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi, L.part[0]);
-        else hipLaunchKernelGGL((k_tv_dual<P, false, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi, L.part[0]);
+        if (gap) hipLaunchKernelGGL((k_tv_dual<P, true, NT>), dim3(n), dim3(NT), lds, L.stream, L.sl, rows, L.rpi, L.part[0]);
+        else hipLaunchKernelGGL((k_tv_dual<P, false, NT>), dim3(n), dim3(NT), lds, L.stream, L.sl, rows, L.rpi, L.part[0]);
       } else {
-        if (gap) hipLaunchKernelGGL((k_tv_primal<P, true, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi, L.part[1]);
-        else hipLaunchKernelGGL((k_tv_primal<P, false, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi, L.part[1]);
+        if (gap) hipLaunchKernelGGL((k_tv_primal<P, true, NT>), dim3(n), dim3(NT), lds, L.stream, L.sl, rows, L.rpi, L.part[1]);
+        else hipLaunchKernelGGL((k_tv_primal<P, false, NT>), dim3(n), dim3(NT), lds, L.stream, L.sl, rows, L.rpi, L.part[1]);
       }
     } else if (stage == 0) {
-      hipLaunchKernelGGL((k_tgv_primal<P, S, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi);
+      hipLaunchKernelGGL((k_tgv_primal<P, S, NT>), dim3(n), dim3(NT), lds, L.stream, L.sl, rows, L.rpi);
     } else if (gap) {
-      hipLaunchKernelGGL((k_tgv_dual<P, S, true, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi, L.part[0]);
+      hipLaunchKernelGGL((k_tgv_dual<P, S, true, NT>), dim3(n), dim3(NT), lds, L.stream, L.sl, rows, L.rpi, L.part[0]);
     } else {
-      hipLaunchKernelGGL((k_tgv_dual<P, S, false, NT>), dim3(n), dim3(NT), 0, L.stream, L.sl, rows, L.rpi, L.part[0]);
+      hipLaunchKernelGGL((k_tgv_dual<P, S, false, NT>), dim3(n), dim3(NT), lds, L.stream, L.sl, rows, L.rpi, L.part[0]);
     }
   };
   if (L.den_threads >= 1024) launch(std::integral_constant<int, 1024>{});
@@ -1858,13 +1875,14 @@ bool fft_len_compiled(int dtype, int n) {
 template <class R, int MODE, int NF>
 static void launch_rows_adj(const Launch& L, bool gap) {
   constexpr int NR = kRowThreads, NW = kRowAdjThreadsWide;
+  const size_t lds = lds_occupancy(L.lds_rows_a, L.wpc[WPC_ADJ], init_kernel_attrs());
   if (L.adj_threads == NW) {
     const int nv = L.dense ? L.nvstep : 0;
-    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NW, NF>), L.nrg_a + nv, NW, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, nv);
-    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NW, NF>), L.nrg_a + nv, NW, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, nv);
+    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NW, NF>), L.nrg_a + nv, NW, lds, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, nv);
+    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NW, NF>), L.nrg_a + nv, NW, lds, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, nv);
   } else {
-    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NR, NF>), L.nrg_a, NR, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, 0);
-    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NR, NF>), L.nrg_a, NR, L.lds_rows_a, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, 0);
+    if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NR, NF>), L.nrg_a, NR, lds, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, 0);
+    else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NR, NF>), L.nrg_a, NR, lds, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, 0);
   }
 }
 
@@ -1874,18 +1892,20 @@ template <class R, int MODE>
 static void mri_stage_m(const Launch& L, bool gap, int stage) {
   constexpr int NR = kRowThreads, NC = kColThreads;
   if (stage == 0) {
+    const size_t lds = lds_occupancy(L.lds_rows, L.wpc[WPC_FWD], init_kernel_attrs());
     with_fft_len<R>(L.fft_w, [&](auto nf) {
       constexpr int NF = decltype(nf)::value;
       const int nd = L.dense ? L.ndual : 0;
-      if (gap) PDT_LAUNCH_NT((k_rows_fwd<R, MODE, true, NR, NF>), L.nrg + nd, NR, L.lds_rows, L.stream, L.sl, L.dense ? nullptr : L.rowgroups, L.nrg, L.cb, L.part[0], nd);
-      else PDT_LAUNCH_NT((k_rows_fwd<R, MODE, false, NR, NF>), L.nrg + nd, NR, L.lds_rows, L.stream, L.sl, L.dense ? nullptr : L.rowgroups, L.nrg, L.cb, L.part[0], nd);
+      if (gap) PDT_LAUNCH_NT((k_rows_fwd<R, MODE, true, NR, NF>), L.nrg + nd, NR, lds, L.stream, L.sl, L.dense ? nullptr : L.rowgroups, L.nrg, L.cb, L.part[0], nd);
+      else PDT_LAUNCH_NT((k_rows_fwd<R, MODE, false, NR, NF>), L.nrg + nd, NR, lds, L.stream, L.sl, L.dense ? nullptr : L.rowgroups, L.nrg, L.cb, L.part[0], nd);
     });
   } else if (stage == 1) {
     if (L.ncolitems == 0) return;
+    const size_t lds = lds_occupancy(L.lds_cols, L.wpc[WPC_COLS], init_kernel_attrs());
     with_fft_len<R>(L.fft_h, [&](auto nf) {
       constexpr int NF = decltype(nf)::value;
-      if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.dense ? nullptr : L.cols, L.ncolitems, L.cg, L.part[1]);
-      else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.dense ? nullptr : L.cols, L.ncolitems, L.cg, L.part[1]);
+      if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, L.dense ? nullptr : L.cols, L.ncolitems, L.cg, L.part[1]);
+      else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, L.dense ? nullptr : L.cols, L.ncolitems, L.cg, L.part[1]);
     });
   } else if (stage == 2) {
     with_fft_len<R>(L.fft_w, [&](auto nf) {

@@ -51,7 +51,20 @@ struct Launch {
   // workgroup size of the denoising half-iterations (256, 512 or 1024): the
   // smallest that covers rpi rows of the widest slice in one pass, at most 1024
   int den_threads = 256;
+  // occupancy caps, workgroups per CU (0: none) of the forward row pass, the
+  // column pass, the adjoint row pass and the denoising half-iterations: a
+  // launch asks for enough LDS that no more than that many workgroups fit
+  // the LDS of a CU (the autotuner's waves-per-CU axis)
+  int wpc[4] = {0, 0, 0, 0};
 };
+enum { WPC_FWD = 0, WPC_COLS = 1, WPC_ADJ = 2, WPC_DEN = 3 };
+
+// Dynamic LDS bytes of a launch that needs `need` bytes and is capped at
+// wpc workgroups per CU (0: no cap): at least `need`, raised to the share of
+// the CU's LDS that leaves room for no more than wpc workgroups.  Caps that
+// would need more than `limit` bytes (the dynamic LDS the kernel may ask
+// for) are not applied.
+size_t lds_occupancy(size_t need, int wpc, size_t limit);
 
 // Block size of k_mri_primal covering rows of width W in one pass.
 inline int primal_block(int W) { return W >= 1024 ? 1024 : ((W + 63) / 64) * 64; }

@@ -28,8 +28,9 @@ bool g_loaded = false;
 
 const char* cache_path() { return std::getenv("PDT_TUNE_CACHE"); }
 
-// File format: one entry per line, "kind dt W H Hk C nc ns : cb cg rpi [cb_a]"
-// (cb_a absent: the adjoint row pass uses cb).
+// File format: one entry per line,
+// "kind dt W H Hk C nc ns : cb cg rpi [cb_a [wpc0 wpc1 wpc2 wpc3]]"
+// (cb_a absent: the adjoint row pass uses cb; occupancy caps absent: none).
 void load_locked() {
   if (g_loaded) return;
   g_loaded = true;
@@ -45,6 +46,9 @@ void load_locked() {
     if (ss >> k.kind >> k.dt >> k.W >> k.H >> k.Hk >> k.C >> k.nc >> k.ns >> colon >> v.cb >> v.cg >> v.rpi &&
         colon == ':') {
       if (!(ss >> v.cb_a)) v.cb_a = 0;
+      int w[4];
+      if (ss >> w[0] >> w[1] >> w[2] >> w[3])
+        for (int i = 0; i < 4; ++i) v.wpc[i] = w[i];
       g_cache[k] = v;
     }
   }
@@ -56,7 +60,8 @@ void save_locked() {
   std::ofstream out(p, std::ios::trunc);
   for (const auto& kv : g_cache)
     out << kv.first.str() << " : " << kv.second.cb << ' ' << kv.second.cg << ' ' << kv.second.rpi << ' '
-        << kv.second.cb_a << '\n';
+        << kv.second.cb_a << ' ' << kv.second.wpc[0] << ' ' << kv.second.wpc[1] << ' ' << kv.second.wpc[2] << ' '
+        << kv.second.wpc[3] << '\n';
 }
 }  // namespace
 

@@ -2,9 +2,10 @@
 //
 // Tiles are keyed by the problem shape; values are the coils per LDS chunk
 // of the forward (cb) and adjoint (cb_a) row kernels, the columns per group
-// (cg) of the column kernel and the rows per item (rpi) of the denoising
-// kernels.  A value of 0 means "use
-// the built-in default".  The cache is process-wide and, when the environment
+// (cg) of the column kernel, the rows per item (rpi) of the denoising
+// kernels and the occupancy caps (workgroups per CU) of the forward row,
+// column, adjoint row and denoising passes.  A value of 0 means "use the
+// built-in default" (no cap, for the occupancy).  The cache is process-wide and, when the environment
 // variable PDT_TUNE_CACHE names a file, loaded from / saved to that file, so a
 // tuned choice is deterministic across runs.
 #pragma once
@@ -21,6 +22,7 @@ struct TuneKey {
 
 struct TuneVal {
   int cb = 0, cg = 0, rpi = 0, cb_a = 0;
+  int wpc[4] = {0, 0, 0, 0};  // workgroups per CU of the passes (0: no cap)
 };
 
 TuneVal tune_lookup(const TuneKey& k);

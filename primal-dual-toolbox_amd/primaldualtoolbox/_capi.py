@@ -29,7 +29,7 @@ EXPORTS = [
     "pdt_solver_create", "pdt_params_default", "pdt_solver_get_params", "pdt_solver_set_params",
     "pdt_solver_set_operator", "pdt_solver_set_noisy_data", "pdt_solver_set_input0",
     "pdt_solver_solve", "pdt_solver_get_result", "pdt_solver_get_v", "pdt_solver_get_stats",
-    "pdt_solver_destroy", "pdt_solver_profile", "pdt_autotune", "pdt_set_tiles", "pdt_batch_solve",
+    "pdt_solver_destroy", "pdt_solver_profile", "pdt_batch_profile", "pdt_autotune", "pdt_set_tiles", "pdt_batch_solve",
     "pdt_cnn_create", "pdt_cnn_destroy", "pdt_cnn_num_params", "pdt_cnn_set_params", "pdt_cnn_get_params",
     "pdt_cnn_forward", "pdt_cnn_loss_grad", "pdt_cnn_adam_step", "pdt_cnn_train_step", "pdt_solver_warm_start",
 ]
@@ -45,7 +45,7 @@ class PdtParams(ct.Structure):
 class PdtStats(ct.Structure):
     _fields_ = [("iters", ct.c_uint32), ("n_gap", ct.c_uint32), ("primal", ct.c_double),
                 ("dual", ct.c_double), ("gap", ct.c_double), ("rel_gap", ct.c_double),
-                ("ms", ct.c_double), ("converged", ct.c_int32)]
+                ("ms", ct.c_double), ("converged", ct.c_int32), ("ms_to_stop", ct.c_double)]
 
 
 class PdtKernelProf(ct.Structure):
@@ -109,9 +109,11 @@ def lib():
         L.pdt_solver_destroy.argtypes = [vp]
         L.pdt_solver_profile.argtypes = [vp, ct.c_int, ct.POINTER(PdtKernelProf), ct.c_int,
                                          ct.POINTER(ct.c_int)]
+        L.pdt_batch_profile.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ct.POINTER(PdtKernelProf), ct.c_int,
+                                         ct.POINTER(ct.c_int)]
         ip = ct.POINTER(ct.c_int)
-        L.pdt_autotune.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ip, ip, ip, ip, dp]
-        L.pdt_set_tiles.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int]
+        L.pdt_autotune.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ip, ip, ip, ip, ip, dp]
+        L.pdt_set_tiles.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ip]
         L.pdt_batch_solve.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int]
         fp, szp = ct.POINTER(ct.c_float), ct.POINTER(ct.c_size_t)
         L.pdt_cnn_create.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(vp)]

@@ -251,27 +251,45 @@ class _Optimizer:
 
 
 def autotune(optimizers, reps: int = 20) -> dict:
-    """Times the candidate tile shapes on these configured optimizers (one
-    batch), caches the fastest and returns it.  Leaves the state advanced:
-    reset with setInput0 / setNoisyData before solving."""
+    """Times the candidate tile shapes, then occupancy caps (workgroups per
+    CU) per pass, on these configured optimizers (one batch), caches the
+    fastest and returns it.  Leaves the state advanced: reset with
+    setInput0 / setNoisyData before solving."""
     for o in optimizers:
         o._push_params()
     arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
     cb, cg, rpi, cb_a, ms = ct.c_int(), ct.c_int(), ct.c_int(), ct.c_int(), ct.c_double()
+    wpc = (ct.c_int * 4)()
     C.check(C.lib().pdt_autotune(arr, len(optimizers), int(reps), ct.byref(cb), ct.byref(cg), ct.byref(rpi),
-                                 ct.byref(cb_a), ct.byref(ms)))
-    return {"cb": cb.value, "cg": cg.value, "rpi": rpi.value, "cb_a": cb_a.value, "ms_per_iteration": ms.value}
+                                 ct.byref(cb_a), wpc, ct.byref(ms)))
+    return {"cb": cb.value, "cg": cg.value, "rpi": rpi.value, "cb_a": cb_a.value, "wg_per_cu": list(wpc),
+            "ms_per_iteration": ms.value}
 
 
-def set_tiles(optimizers, cb: int = 0, cg: int = 0, rpi: int = 0, cb_a: int = 0) -> None:
+def set_tiles(optimizers, cb: int = 0, cg: int = 0, rpi: int = 0, cb_a: int = 0, wg_per_cu=None) -> None:
     """Stores a manual tile choice for the shape of these optimizers (one
     batch) in the tile cache the autotuner uses: coils per forward and
     adjoint row item (cb, cb_a), columns per column item (cg), rows per
-    denoising item (rpi); 0 takes the default."""
+    denoising item (rpi), 0 = the default; wg_per_cu: 4 occupancy caps
+    (forward row, column, adjoint row, denoising passes; 0 = none)."""
     for o in optimizers:
         o._push_params()
     arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
-    C.check(C.lib().pdt_set_tiles(arr, len(optimizers), int(cb), int(cg), int(rpi), int(cb_a)))
+    wpc = None if wg_per_cu is None else (ct.c_int * 4)(*[int(x) for x in wg_per_cu])
+    C.check(C.lib().pdt_set_tiles(arr, len(optimizers), int(cb), int(cg), int(rpi), int(cb_a), wpc))
+
+
+def batch_profile(optimizers, reps: int = 20) -> list:
+    """profileKernels over the launches of a batch of configured optimizers:
+    per fused kernel, ms per launch over every slice and the algorithmic bytes
+    of that launch summed over the slices.  Advances the states."""
+    for o in optimizers:
+        o._push_params()
+    arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
+    out = (C.PdtKernelProf * 8)()
+    n = ct.c_int()
+    C.check(C.lib().pdt_batch_profile(arr, len(optimizers), int(reps), out, 8, ct.byref(n)))
+    return [{"name": out[i].name.decode(), "ms": out[i].ms, "bytes": out[i].bytes} for i in range(n.value)]
 
 
 def batch_solve(optimizers, verbose: bool = False) -> None:

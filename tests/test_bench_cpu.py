@@ -33,3 +33,19 @@ def test_help_lists_the_options():
     assert r.returncode == 0
     for opt in ("--only", "--no-graph", "--no-autotune", "--fixed-iters", "--calibrate-copy", "--no-warm"):
         assert opt in r.stdout
+
+
+def test_cpu_baseline_threads_and_host_record(monkeypatch):
+    """The CPU baseline uses OMP_NUM_THREADS where the host sets it (the
+    CPU share of this GPU), else every physical core, and records the host."""
+    import importlib
+    sys.path.insert(0, ROOT)
+    bench = importlib.import_module("bench")
+    monkeypatch.setenv("OMP_NUM_THREADS", "5")
+    assert bench.default_cpu_threads() == 5
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    info = bench.host_info()
+    assert bench.default_cpu_threads() == info.get("physical_cores", os.cpu_count())
+    assert info["nproc"] == os.cpu_count() and "model" in info
+    med, ts = bench.timed_cpu(iter([3.0, 1.0, 2.0, 5.0, 4.0, 9.0]).__next__)
+    assert (med, ts) == (4.0, [1.0, 2.0, 5.0, 4.0, 9.0])

@@ -196,9 +196,10 @@ def test_batch_checks_fire_before_any_launch():
         for hs, msg in cases:
             assert L.pdt_batch_solve(hs, len(hs), 0) == C.PDT_ERR_ARG
             assert msg.encode() in L.pdt_last_error()
-            assert L.pdt_autotune(hs, len(hs), 1, *[ct.byref(o) for o in out], ct.byref(ms)) == C.PDT_ERR_ARG
+            wpc = (ct.c_int * 4)()
+            assert L.pdt_autotune(hs, len(hs), 1, *[ct.byref(o) for o in out], wpc, ct.byref(ms)) == C.PDT_ERR_ARG
             assert msg.encode() in L.pdt_last_error()
-            assert L.pdt_set_tiles(hs, len(hs), 1, 1, 1, 1) == C.PDT_ERR_ARG
+            assert L.pdt_set_tiles(hs, len(hs), 1, 1, 1, 1, None) == C.PDT_ERR_ARG
             assert msg.encode() in L.pdt_last_error()
         # a consistent batch gets past the checks to the reference's state error
         assert L.pdt_batch_solve(arr(a, d), 2, 0) == C.PDT_ERR_STATE

@@ -180,3 +180,28 @@ def test_autotune_cache_file_is_deterministic_and_tuned_solve_matches_oracle(tmp
     ref, _, _, _ = oracle.solve(oracle.TGV_MRI, case["kspace"].astype(np.complex128), op=oop, alpha1=a1,
                                 alpha0=2 * a1, max_iter=39)
     assert rel_l2(u1, ref) < 1e-3
+
+
+@pytest.mark.parametrize("kind", ["mri", "tgv"])
+def test_occupancy_caps_change_only_the_schedule(kind):
+    """A cap on the workgroups per CU (extra LDS per workgroup) leaves the
+    kernels and tiles as they were: the iterates are bit for bit those
+    without a cap."""
+    import primaldualtoolbox as pdt
+    from primaldualtoolbox import synthetic as syn
+    res = []
+    for caps in (None, [1, 2, 1, 3], [3, 1, 2, 4]):
+        if kind == "mri":
+            case = syn.mri_case(seed=12, n=96, coils=9, center=8)
+            _, opt, _ = _solver(case, 30)
+        else:
+            _, noisy = syn.tgv_denoising_case(seed=3, n=200)
+            opt = pdt.denoising.TgvOptimizer_2c()
+            opt.setNoisyData(noisy)
+            p = opt.getParameters()
+            p.Lambda, p.alpha0, p.alpha1, p.max_iter = 20, 2, 1, 30
+        pdt.mri.set_tiles([opt], cb=4, cg=2, cb_a=4, wg_per_cu=caps)
+        opt.solve()
+        res.append(opt.getResult().copy())
+    np.testing.assert_array_equal(res[0], res[1])
+    np.testing.assert_array_equal(res[0], res[2])
