@@ -216,7 +216,7 @@ def _kernels(prof) -> dict:
                         "GBs": round(p["bytes"] / (p["ms"] * 1e-3) / 1e9, 1)} for p in prof}
 
 
-def extra_tv256(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
+def extra_tv256(steps: int, cpu_threads: int, no_cpu: bool, autotune: bool = True) -> dict:
     """ROF denoising, 256^2 float32, lambda 10, 500 iterations.  Accuracy is
     the relative L2 distance to the FP64 GPU solver on the same input."""
     import primaldualtoolbox as pdt
@@ -224,19 +224,24 @@ def extra_tv256(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
     from primaldualtoolbox import synthetic as syn
     _, noisy = syn.tv_denoising_case(seed=0, n=256, sigma=0.1)
     res = {}
+    tiles = None
     for name, dt in (("f32", np.float32), ("f64", np.float64)):
         opt = pdt.denoising.TvOptimizer_2f(dtype=dt)
         opt.setNoisyData(noisy.astype(dt))
         p = opt.getParameters()
         p.Lambda, p.max_iter, p.gap_every = 10, 499, 500
         f_dev = C.DeviceArray.from_numpy(noisy.astype(dt), dt)
+        if autotune:
+            t = pdt.denoising.autotune([opt], reps=20)
+            tiles = t if name == "f32" else tiles
+            opt.setInput0(noisy.astype(dt))
         opt.solve()
         res[name] = (opt, _timed_solves(opt, f_dev, steps), opt.getResult().copy())
     opt, s, u = res["f32"]
     st = opt.getStats()
     out = {"workload": "TV-L2 (ROF) 256x256 float32, lambda=10, 500 iterations",
            "seconds": s, "us_per_iteration": s * 1e6 / 500, "rel_gap": float(st["rel_gap"]),
-           "rel_l2_vs_f64": _rel(u, res["f64"][2]), "seconds_f64": res["f64"][1],
+           "rel_l2_vs_f64": _rel(u, res["f64"][2]), "seconds_f64": res["f64"][1], "tiles": tiles,
            "kernels": _kernels(opt.profileKernels(50))}
     if not no_cpu:
         O = _oracle()
@@ -459,7 +464,7 @@ def side_measurements(args, keys=None) -> dict:
     """The side measurements named in keys (all when None): cnn, tv_256,
     tgv_512, demo_384, batch (the batch of args.batch slices)."""
     runs = {"cnn": extra_cnn,
-            "tv_256": lambda: extra_tv256(3, args.cpu_threads, args.no_cpu),
+            "tv_256": lambda: extra_tv256(3, args.cpu_threads, args.no_cpu, not args.no_autotune),
             "tgv_512": lambda: extra_tgv512(3, args.cpu_threads, args.no_cpu),
             "demo_384": lambda: extra_demo384(3, args.cpu_threads, args.no_cpu, not args.no_autotune,
                                               not args.no_graph),

@@ -210,28 +210,38 @@ PDT_API int pdt_solver_profile(pdt_solver* s, int reps, pdt_kernel_prof* out, in
  * algorithmic bytes of that launch summed over the slices. */
 PDT_API int pdt_batch_profile(pdt_solver** solvers, int count, int reps, pdt_kernel_prof* out, int cap, int* n);
 
-/* NEW: autotuner.  Times every candidate tile shape (coils per LDS chunk of
- * the forward and of the adjoint row pass, columns per group of the column
- * pass, rows per work item of the denoising passes) on the configured solvers
- * (one batch) with hipEvents; then, with those tiles, caps of 1, 2, 3, 4 or
- * 6 workgroups per CU (3, 4 or 6 for denoising) on each pass in turn, each
- * pass timed alone, keeping a cap only where it beats no cap by 2 %.  Keeps
- * the choice in the process-wide tile cache (persisted to the file named by
- * the environment variable PDT_TUNE_CACHE, if set) and returns it: wpc
- * (4 ints, may be NULL) receives the caps of the forward row, column,
- * adjoint row and denoising passes (0: none); ms_per_iteration the stage
- * times of the choice.  Advances the solver state: call before the solves
- * and reset with set_input0. */
-PDT_API int pdt_autotune(pdt_solver** solvers, int count, int reps, int* cb, int* cg, int* rpi, int* cb_a,
-                         int* wpc, double* ms_per_iteration);
+/* NEW: tile shapes and launch shapes of the fused kernels, one choice per
+ * problem shape (kept in a process-wide cache, persisted to the file named by
+ * the environment variable PDT_TUNE_CACHE, if set). */
+typedef struct pdt_tiles {
+  int32_t cb;      /* coils per LDS chunk of the forward row pass (MRI)           */
+  int32_t cg;      /* columns per group of the column pass (MRI)                 */
+  int32_t rpi;     /* rows per work item of the denoising passes                 */
+  int32_t cb_a;    /* coils per LDS chunk of the adjoint row pass (MRI)          */
+  int32_t wpc[4];  /* workgroups per CU, 0 = no cap: forward row, column,
+                      adjoint row and denoising passes                          */
+  int32_t tb_x, tb_y, tb_k; /* ROF, one slice: tb_k iterations per launch on
+                      tb_x x tb_y tiles held in LDS (tb_k 0: one iteration per
+                      pair of launches)                                          */
+} pdt_tiles;
+
+/* NEW: autotuner.  Times every candidate tile shape (the fields of
+ * pdt_tiles; for a single ROF slice also temporally blocked launches) on the
+ * configured solvers (one batch) with hipEvents, per iteration; then, with
+ * those tiles, caps of 1, 2, 3, 4 or 6 workgroups per CU (3, 4 or 6 for
+ * denoising) on each pass in turn, each pass timed alone, keeping a cap only
+ * where it beats no cap by 2 %.  Stores the choice in the tile cache and
+ * returns it (out may be NULL) with the time per iteration it gives.
+ * Advances the solver state: call before the solves and reset with
+ * set_input0. */
+PDT_API int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, double* ms_per_iteration);
 
 /* NEW: a manual tile choice for the shape of these solvers (one batch),
- * stored in the same tile cache as pdt_autotune's result; values <= 0 take
- * the default, values above a shape's limit are clamped.  wpc (4 ints or
- * NULL = no caps): workgroups per CU of the forward row, column, adjoint row
- * and denoising passes, 0 = no cap, at most 6.  The next solve of the same
- * shape uses them. */
-PDT_API int pdt_set_tiles(pdt_solver** solvers, int count, int cb, int cg, int rpi, int cb_a, const int* wpc);
+ * stored in the same tile cache as pdt_autotune's result: cb, cg, rpi, cb_a
+ * <= 0 take the default, values above a shape's limit are clamped; wpc in
+ * [0, 6]; tb_k <= 0 turns the blocked ROF launches off, and tiles that do
+ * not fit the LDS are not used.  The next solve of the same shape uses it. */
+PDT_API int pdt_set_tiles(pdt_solver** solvers, int count, const pdt_tiles* tiles);
 
 /* NEW: solve many configured solvers (same kind and dtype) together, with one
  * persistent work-list launch per half-iteration for the whole batch.  Each

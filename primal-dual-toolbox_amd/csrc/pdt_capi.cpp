@@ -200,8 +200,10 @@ int max_cg(int dt, int Hk, int nc) {
   return cg;
 }
 // Default tiles before any autotuning: 4 coils per row item, 2 columns per
-// column group.
+// column group; temporally blocked ROF launches of 8 iterations on 32 x 16
+// tiles.
 constexpr int kDefaultCb = 4, kDefaultCg = 2;
+constexpr int kDefaultTvBlock[3] = {32, 16, 8};
 
 }  // namespace
 
@@ -617,6 +619,7 @@ struct pdt_solver {
   bool have_u = false;
   std::unique_ptr<pdt_array> u, v;  // result handles (u: {W,H}, v: {W,H,2})
   DevBuf ub, vb, p, q, tmp;
+  DevBuf u2, ub2, p2;  // ROF: second copy of u, ubar, p for the temporally blocked launches
   DevBuf fk, rk, T1, T2, wpart;
   // set-up buffers kept across solves: descriptor and item list of the data
   // compaction, its partial sums, A^*f, and two reduction results
@@ -686,6 +689,11 @@ void alloc_state(pdt_solver* s, int W, int H) {
     exact(s->vb, 2 * n * e);
     exact(s->q, 3 * n * s->pq_e());
     if (s->kind == PDT_SOLVER_TGV) exact(s->tmp, 2 * n * e);
+  }
+  if (s->kind == PDT_SOLVER_TV) {
+    exact(s->u2, n * e);
+    exact(s->ub2, n * e);
+    exact(s->p2, 2 * n * s->pq_e());
   }
   s->W = W;
   s->H = H;
@@ -785,6 +793,7 @@ struct Engine {
   bool fp16 = false;
   int force_cb = 0, force_cg = 0, force_rpi = 0, force_cb_a = 0;  // autotuner overrides (0: cache / default)
   int force_wpc[4] = {-1, -1, -1, -1};  // occupancy caps (-1: cache; 0: none)
+  int force_tb[3] = {0, 0, -1};         // blocked ROF tiles (tb[2] -1: cache / default; 0: off)
   bool plan_only = false;                         // build(): stop after the signature
   std::string sig;
   TuneKey key{};
@@ -867,6 +876,14 @@ struct Engine {
       TuneVal tv = tune_lookup(key);
       for (int i = 0; i < 4; ++i) L.wpc[i] = force_wpc[i] >= 0 ? force_wpc[i] : tv.wpc[i];
       rpi = force_rpi > 0 ? force_rpi : tv.rpi > 0 ? tv.rpi : 1;
+      // temporally blocked ROF launches: one slice only, tiles that fit the LDS
+      const int* tb = force_tb[2] >= 0 ? force_tb : tv.tb[2] >= 0 ? tv.tb : kDefaultTvBlock;
+      if (kind == PDT_SOLVER_TV && ns == 1 && tb[2] > 0 && tb[0] > 0 && tb[1] > 0 &&
+          tv_block_lds(dt, tb[0], tb[1], tb[2]) <= init_kernel_attrs()) {
+        L.tb_x = tb[0];
+        L.tb_y = tb[1];
+        L.tb_k = tb[2];
+      }
     }
     // slices, grouped by FFT lengths, and their work lists
     hs.assign(ns, SliceDev{});
@@ -920,6 +937,9 @@ struct Engine {
         d.q = s->q.p;
         d.tmp = s->tmp.p;
       }
+      d.u2 = s->u2.p;
+      d.ub2 = s->ub2.p;
+      d.p2 = s->p2.p;
       d.f = s->f->ptr;
       if (mri) {
         d.fk = s->fk.p;
@@ -964,7 +984,8 @@ struct Engine {
     // signature of everything the kernels and graphs depend on (the control
     // and statistics words are owned by the engine and added below)
     sig.assign(reinterpret_cast<const char*>(hs.data()), hs.size() * sizeof(SliceDev));
-    const int tiles[9] = {cb, cg, rpi, int(sv[0]->prm.gap_every), cb_a, L.wpc[0], L.wpc[1], L.wpc[2], L.wpc[3]};
+    const int tiles[12] = {cb,       cg,       rpi,      int(sv[0]->prm.gap_every), cb_a,   L.wpc[0],
+                           L.wpc[1], L.wpc[2], L.wpc[3], L.tb_x,                    L.tb_y, L.tb_k};
     sig.append(reinterpret_cast<const char*>(tiles), sizeof(tiles));
     if (plan_only) return;
     cs_bytes = size_t(ns) * (ST_WORDS * sizeof(double) + CTL_WORDS * sizeof(int));
@@ -1136,6 +1157,9 @@ struct Engine {
       x.dense = dense;
       x.ndual = ndual;
       x.nvstep = nvstep;
+      if (!(dense && kind == PDT_SOLVER_TV)) x.tb_x = x.tb_y = x.tb_k = 0;
+      x.tb_w = hs[groups[r.g][0]].W;
+      x.tb_h = hs[groups[r.g][0]].H;
       listed_items += r.nrow + r.nrg + r.nrga + r.ncol;
       G.push_back(x);
     }
@@ -1219,9 +1243,28 @@ struct Engine {
     HIPC(hipMemcpyAsync(d_cs.p, h_cs, cs_bytes, hipMemcpyHostToDevice, stream));
   }
 
+  bool blocked() const { return G.size() == 1 && G[0].tb_k > 0; }
+
   // Enqueues `len` iterations; the last one evaluates the gap when asked.
+  // With temporally blocked ROF launches, the iterations before the gap
+  // iteration run as pairs of launches of tb_k iterations each (the state
+  // goes to the second copy and back), the rest one iteration at a time.
   void enqueue(int len, bool gap) {
     const int dt = dtk();
+    if (blocked()) {
+      const int k = G[0].tb_k;
+      int plain = len - (gap ? 1 : 0);
+      for (; plain >= 2 * k; plain -= 2 * k) {
+        run_tv_block(G[0], dt, 0);
+        run_tv_block(G[0], dt, 1);
+      }
+      for (int i = 0; i < plain; ++i) iteration(false);
+      if (gap) {
+        iteration(true);
+        run_gap_reduce(L, kind, dt);
+      }
+      return;
+    }
     for (int i = 0; i < len; ++i) {
       const bool g = gap && i == len - 1;
       iteration(g);
@@ -1268,7 +1311,13 @@ struct Engine {
     // Graphs hold at most kChunk iterations (a long graph's launch latency is
     // not hidden); the gap is evaluated at the multiples of gap_every.
     constexpr unsigned kChunk = 50;
-    const unsigned chunk = std::min<unsigned>(gap ? gap_every : total, kChunk);
+    unsigned chunk = std::min<unsigned>(gap ? gap_every : total, kChunk);
+    // blocked ROF launches come in pairs of tb_k iterations: chunks of a
+    // multiple of 2 tb_k leave no iterations to the one-iteration kernels
+    if (blocked()) {
+      const unsigned pair = 2u * unsigned(G[0].tb_k);
+      chunk = std::max(pair, chunk - chunk % pair);
+    }
     const bool use_graph = sv[0]->prm.use_graph != 0;
     std::vector<double> stop_ms(ns, -1.0);  // batched solves: time from the start to each slice's stop
     reset_ctl();
@@ -1887,6 +1936,44 @@ void profile_engine(std::vector<pdt_solver*> sv, int reps, pdt_kernel_prof* out,
     e.sv = sv;
     e.build();
     e.reset_ctl();
+    if (e.blocked()) {
+      // temporally blocked ROF launches, as a solve runs them: a graph of
+      // reps pairs of launches timed with an event pair; per launch (tb_k
+      // iterations): read u, ubar, p, f, write u, ubar, p once each
+      hipEvent_t e0, e1;
+      HIPC(hipEventCreate(&e0));
+      HIPC(hipEventCreate(&e1));
+      hipGraph_t graph;
+      HIPC(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+      for (int r = 0; r < reps; ++r) {
+        run_tv_block(e.G[0], e.dtk(), 0);
+        run_tv_block(e.G[0], e.dtk(), 1);
+      }
+      HIPC(hipStreamEndCapture(e.stream, &graph));
+      hipGraphExec_t ge;
+      HIPC(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      HIPC(hipGraphLaunch(ge, e.stream));
+      HIPC(hipEventRecord(e0, e.stream));
+      HIPC(hipGraphLaunch(ge, e.stream));
+      HIPC(hipEventRecord(e1, e.stream));
+      HIPC(hipEventSynchronize(e1));
+      (void)hipGraphExecDestroy(ge);
+      float loop_ms = 0;
+      HIPC(hipEventElapsedTime(&loop_ms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      double bytes = 0;
+      for (auto* s : sv) bytes += 9.0 * double(s->W) * s->H * double(s->pe());
+      *n = 0;
+      if (cap > 0) {
+        std::snprintf(out[0].name, sizeof(out[0].name), "k_tv_block");
+        out[0].ms = double(loop_ms) / (2.0 * reps);
+        out[0].bytes = bytes;
+        *n = 1;
+      }
+      return;
+    }
     // The iteration as a solve runs it: a hipGraph of reps iterations, timed
     // with an event pair on the solver stream (t_it per iteration).  Each
     // stage's share of it comes from event pairs around its direct launches
@@ -1988,14 +2075,34 @@ std::vector<double> time_stages(Engine& e, int reps) {
   return out;
 }
 double time_iteration(Engine& e, int reps) {
+  if (e.blocked()) {
+    // pairs of temporally blocked launches, per iteration
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    run_tv_block(e.G[0], e.dtk(), 0);
+    run_tv_block(e.G[0], e.dtk(), 1);
+    HIPC(hipEventRecord(e0, e.stream));
+    for (int r = 0; r < reps; ++r) {
+      run_tv_block(e.G[0], e.dtk(), 0);
+      run_tv_block(e.G[0], e.dtk(), 1);
+    }
+    HIPC(hipEventRecord(e1, e.stream));
+    HIPC(hipEventSynchronize(e1));
+    HIPC(hipGetLastError());
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return double(ms) / (2.0 * reps * e.G[0].tb_k);
+  }
   double tot = 0;
   for (double t : time_stages(e, reps)) tot += t;
   return tot;
 }
 }  // namespace
 
-int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* out_cg, int* out_rpi, int* out_cb_a,
-                 int* out_wpc, double* out_ms) {
+int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, double* out_ms) {
   return guard([&] {
     if (!solvers || count <= 0 || reps <= 0) fail(PDT_ERR_ARG, "bad argument");
     std::vector<pdt_solver*> sv(solvers, solvers + count);
@@ -2005,8 +2112,14 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* ou
     // Phase 1, tile shapes without occupancy caps: coils per forward row
     // item, columns per column item, rows per denoising item, coils per
     // adjoint row item (the forward choice, or every coil of the row, which
-    // also fuses the u step into that pass)
-    std::vector<std::array<int, 4>> cand;
+    // also fuses the u step into that pass); for a single ROF slice also
+    // temporally blocked launches (interior tile, iterations per launch),
+    // compared per iteration
+    struct Cand {
+      std::array<int, 4> t;
+      std::array<int, 3> tb;
+    };
+    std::vector<Cand> cand;
     if (mri) {
       int maxC = 1;
       for (auto* s : sv) maxC = std::max(maxC, s->op->C);
@@ -2016,19 +2129,25 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* ou
       for (int cb : cbs)
         if (cb <= maxC)
           for (int cg : {1, 2, 4}) {
-            cand.push_back({cb, cg, 0, cb});
-            if (cb != maxC) cand.push_back({cb, cg, 0, maxC});
+            cand.push_back({{cb, cg, 0, cb}, {0, 0, 0}});
+            if (cb != maxC) cand.push_back({{cb, cg, 0, maxC}, {0, 0, 0}});
           }
     } else {
-      for (int rpi : {1, 2, 4, 8}) cand.push_back({0, 0, rpi, 0});
+      for (int rpi : {1, 2, 4, 8}) cand.push_back({{0, 0, rpi, 0}, {0, 0, 0}});
+      if (sv[0]->kind == PDT_SOLVER_TV && sv.size() == 1)
+        for (const auto& tb : std::vector<std::array<int, 3>>{
+                 {16, 16, 4}, {16, 16, 6}, {32, 16, 4}, {32, 16, 8}, {32, 32, 8}, {64, 32, 8}, {32, 32, 12},
+                 {64, 64, 8}, {64, 64, 16}})
+          cand.push_back({{0, 0, 1, 0}, tb});
     }
-    auto engine = [&](const std::array<int, 4>& c, const int (&wpc)[4]) {
+    auto engine = [&](const Cand& c, const int (&wpc)[4]) {
       auto e = std::make_unique<Engine>();
       e->sv = sv;
-      e->force_cb = c[0];
-      e->force_cg = c[1];
-      e->force_rpi = c[2];
-      e->force_cb_a = c[3];
+      e->force_cb = c.t[0];
+      e->force_cg = c.t[1];
+      e->force_rpi = c.t[2];
+      e->force_cb_a = c.t[3];
+      for (int i = 0; i < 3; ++i) e->force_tb[i] = c.tb[i];
       for (int i = 0; i < 4; ++i) e->force_wpc[i] = wpc[i];
       e->build();
       e->reset_ctl();
@@ -2036,29 +2155,31 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* ou
     };
     const int none[4] = {0, 0, 0, 0};
     double best = 1e300;
-    std::array<int, 4> bc{0, 0, 0, 0};
+    Cand bc{{0, 0, 0, 0}, {0, 0, 0}};
     std::vector<double> best_st;
     TuneKey key{};
     for (const auto& c : cand) {
       auto e = engine(c, none);
-      const std::vector<double> st = time_stages(*e, reps);
+      if (c.tb[2] > 0 && !e->blocked()) continue;  // tile does not fit
+      const std::vector<double> st = e->blocked() ? std::vector<double>{time_iteration(*e, reps)} : time_stages(*e, reps);
       double ms = 0;
       for (double t : st) ms += t;
       key = e->key;
       if (ms < best) {
         best = ms;
-        bc = {e->L.cb, e->L.cg, e->L.rpi, e->L.cb_a};
+        bc = {{e->L.cb, e->L.cg, e->L.rpi, e->L.cb_a}, {e->G[0].tb_x, e->G[0].tb_y, e->G[0].tb_k}};
         best_st = st;
       }
     }
     // Phase 2, waves per CU: with the chosen tiles, a cap on the workgroups
     // per CU of each pass in turn (its kernel timed alone); a cap is kept
     // only when it beats no cap by 2 %, so timing noise keeps the default.
-    // The denoising stages share one cap (their total is compared).
+    // The denoising stages share one cap (their total is compared); the
+    // blocked ROF launches take none.
     int wpc[4] = {0, 0, 0, 0};
     std::vector<std::pair<int, std::vector<int>>> axes;  // (cap slot, stages it governs)
     if (mri) axes = {{WPC_FWD, {0}}, {WPC_COLS, {1}}, {WPC_ADJ, {2}}};
-    else axes = {{WPC_DEN, {0, 1}}};
+    else if (bc.tb[2] == 0) axes = {{WPC_DEN, {0, 1}}};
     const std::vector<int> caps = mri ? std::vector<int>{1, 2, 3, 4, 6} : std::vector<int>{3, 4, 6};
     for (const auto& ax : axes) {
       auto cost = [&](const std::vector<double>& st) {
@@ -2079,27 +2200,32 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, int* out_cb, int* ou
         }
       }
     }
-    TuneVal tv = mri ? TuneVal{bc[0], bc[1], 0, bc[3]} : TuneVal{0, 0, bc[2], 0};
+    TuneVal tv = mri ? TuneVal{bc.t[0], bc.t[1], 0, bc.t[3]} : TuneVal{0, 0, bc.t[2], 0};
     for (int i = 0; i < 4; ++i) tv.wpc[i] = wpc[i];
+    for (int i = 0; i < 3; ++i) tv.tb[i] = bc.tb[i];
     tune_store(key, tv);
     double ms = 0;
     {
       auto e = engine(bc, wpc);
       ms = time_iteration(*e, reps);
     }
-    if (out_cb) *out_cb = bc[0];
-    if (out_cg) *out_cg = bc[1];
-    if (out_rpi) *out_rpi = bc[2];
-    if (out_cb_a) *out_cb_a = bc[3];
-    if (out_wpc)
-      for (int i = 0; i < 4; ++i) out_wpc[i] = wpc[i];
+    if (out) {
+      out->cb = bc.t[0];
+      out->cg = bc.t[1];
+      out->rpi = bc.t[2];
+      out->cb_a = bc.t[3];
+      for (int i = 0; i < 4; ++i) out->wpc[i] = wpc[i];
+      out->tb_x = bc.tb[0];
+      out->tb_y = bc.tb[1];
+      out->tb_k = bc.tb[2];
+    }
     if (out_ms) *out_ms = ms;
   });
 }
 
-int pdt_set_tiles(pdt_solver** solvers, int count, int cb, int cg, int rpi, int cb_a, const int* wpc) {
+int pdt_set_tiles(pdt_solver** solvers, int count, const pdt_tiles* t) {
   return guard([&] {
-    if (!solvers || count <= 0) fail(PDT_ERR_ARG, "bad argument");
+    if (!solvers || count <= 0 || !t) fail(PDT_ERR_ARG, "bad argument");
     std::vector<pdt_solver*> sv(solvers, solvers + count);
     check_batch(sv);
     for (auto* s : sv) prepare_solve(s, 0);
@@ -2107,15 +2233,21 @@ int pdt_set_tiles(pdt_solver** solvers, int count, int cb, int cg, int rpi, int 
     e.sv = sv;
     e.plan_only = true;
     // zero or negative: the default, not a cached choice
-    e.force_cb = cb > 0 ? cb : kDefaultCb;
-    e.force_cg = cg > 0 ? cg : kDefaultCg;
-    e.force_rpi = rpi > 0 ? rpi : 1;
-    e.force_cb_a = cb_a > 0 ? cb_a : e.force_cb;
-    for (int i = 0; i < 4; ++i) e.force_wpc[i] = wpc ? std::max(0, std::min(wpc[i], 6)) : 0;
+    e.force_cb = t->cb > 0 ? t->cb : kDefaultCb;
+    e.force_cg = t->cg > 0 ? t->cg : kDefaultCg;
+    e.force_rpi = t->rpi > 0 ? t->rpi : 1;
+    e.force_cb_a = t->cb_a > 0 ? t->cb_a : e.force_cb;
+    for (int i = 0; i < 4; ++i) e.force_wpc[i] = std::max(0, std::min(t->wpc[i], 6));
+    e.force_tb[0] = t->tb_x;
+    e.force_tb[1] = t->tb_y;
+    e.force_tb[2] = t->tb_k > 0 ? t->tb_k : 0;
     e.build();
     const bool mri = sv[0]->is_op();
     TuneVal tv = mri ? TuneVal{e.L.cb, e.L.cg, 0, e.L.cb_a} : TuneVal{0, 0, e.L.rpi, 0};
     for (int i = 0; i < 4; ++i) tv.wpc[i] = e.L.wpc[i];
+    tv.tb[0] = e.L.tb_x;
+    tv.tb[1] = e.L.tb_y;
+    tv.tb[2] = e.L.tb_k;
     tune_store(e.key, tv);
   });
 }

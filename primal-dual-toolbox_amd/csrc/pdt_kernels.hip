@@ -867,6 +867,174 @@ __global__ __launch_bounds__(NT) void k_tv_primal(const SliceDev* sl, const int2
   if (ya == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += 1;
 }
 
+// Temporally blocked ROF iterations: tb_k times TV_dual_p_kernel then
+// TV_primal_u_kernel (tgv_kernels.cuh:294-324,568-605) per launch, on a tile
+// held in LDS.  Block (bx, by) owns the interior [x0, x1) x [y0, y1) and
+// loads u, ubar, p (2 planes) and f on the interior grown by k pixels on
+// every side (clipped to the image).  Each iteration shrinks the region
+// where the values are exact by one pixel on each side that is not an image
+// border: the dual step at (x, y) needs ubar at x + 1 and y + 1 (forward
+// differences, none at the last column/row), the primal step p at x - 1 and
+// y - 1 (backward differences, none at the first).  After k iterations the
+// interior is exact and is written to the other copy of the state (src ->
+// dst), so that no block reads a halo that another block of the same launch
+// writes.  The per-pixel arithmetic is that of k_tv_dual / k_tv_primal, in
+// the same order, so the iterates equal theirs bit for bit.
+// Each thread of k_tv_block takes its pixels of a phase in batches of
+// kTvBlockBatch, all loads of a batch issued before its arithmetic, so that
+// the LDS latencies of a batch overlap.
+constexpr int kTvBlockBatch = 4;
+
+template <class P, int NT>
+__global__ __launch_bounds__(NT) void k_tv_block(const SliceDev* sl, int tx, int ty, int k, int dir) {
+  extern __shared__ char smem_raw[];
+  constexpr int B = kTvBlockBatch;
+  const SliceDev& d = sl[0];
+  if (!slice_active(d)) return;
+  using R = typename RealOf<P>::T;
+  const int W = d.W, H = d.H;
+  const int nbx = (W + tx - 1) / tx;
+  const int b = xcd_item(blockIdx.x, gridDim.x);  // neighbouring tiles on one XCD
+  const int by = b / nbx, bx = b - by * nbx;
+  const int x0 = bx * tx, x1 = min(W, x0 + tx), y0 = by * ty, y1 = min(H, y0 + ty);
+  const int xa = max(0, x0 - k), xb = min(W, x1 + k), ya = max(0, y0 - k), yb = min(H, y1 + k);
+  const int LW = xb - xa, LN = LW * (yb - ya);
+  P* sU = reinterpret_cast<P*>(smem_raw);
+  P* sB = sU + LN;
+  P* sX = sB + LN;
+  P* sY = sX + LN;
+  P* sF = sY + LN;
+  const size_t n = size_t(W) * H;
+  {
+    const auto gu = gptr<const P>(dir ? d.u2 : d.u);
+    const auto gb = gptr<const P>(dir ? d.ub2 : d.ub);
+    const auto gp = gptr<const P>(dir ? d.p2 : d.p);
+    const auto gf = gptr<const P>(d.f);
+    const float rLW = 1.0f / float(LW);
+    for (int e0 = threadIdx.x; e0 < LN; e0 += NT * B) {
+      P vu[B], vb[B], vx[B], vy[B], vf[B];
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        const int e = e0 + j * NT;
+        if (e < LN) {
+          const int ly = qdiv(e, LW, rLW), lx = e - ly * LW;
+          const size_t gi = size_t(ya + ly) * W + (xa + lx);
+          vu[j] = gu[gi];
+          vb[j] = gb[gi];
+          vx[j] = gp[gi];
+          vy[j] = gp[gi + n];
+          vf[j] = gf[gi];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        const int e = e0 + j * NT;
+        if (e < LN) {
+          sU[e] = vu[j];
+          sB[e] = vb[j];
+          sX[e] = vx[j];
+          sY[e] = vy[j];
+          sF[e] = vf[j];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const R sigma = R(d.sigma), tau = R(d.tau), lambda = R(d.lambda), theta = R(d.theta);
+  int uxa = xa, uxb = xb, uya = ya, uyb = yb;  // where u, ubar are exact
+  for (int it = 0; it < k; ++it) {
+    // dual step where ubar and its forward neighbours are exact
+    const int pxa = uxa, pxb = uxb == W ? W : uxb - 1, pya = uya, pyb = uyb == H ? H : uyb - 1;
+    {
+      const int rw = pxb - pxa, cnt = rw * (pyb - pya);
+      const float rr = 1.0f / float(max(rw, 1));
+      for (int e0 = threadIdx.x; e0 < cnt; e0 += NT * B) {
+        int l[B];
+        P c[B], ex[B], ey[B], ox[B], oy[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+          const int e = e0 + j * NT;
+          if (e < cnt) {
+            const int ry = qdiv(e, rw, rr), x = pxa + (e - ry * rw), y = pya + ry;
+            l[j] = (y - ya) * LW + (x - xa);
+            c[j] = sB[l[j]];
+            ex[j] = sB[l[j] + (x < W - 1)];
+            ey[j] = sB[l[j] + ((y < H - 1) ? LW : 0)];
+            ox[j] = sX[l[j]];
+            oy[j] = sY[l[j]];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+          if (e0 + j * NT < cnt) {
+            const P tx_ = ex[j] - c[j], ty_ = ey[j] - c[j];
+            const P ax = ox[j] + sigma * tx_;
+            const P ay = oy[j] + sigma * ty_;
+            const R pn = sqrt(abs2(ax) + abs2(ay));
+            const R sp = rcp(fmax(R(1), pn));
+            sX[l[j]] = ax * sp;
+            sY[l[j]] = ay * sp;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // primal step where p and its backward neighbours are exact
+    uxa = pxa == 0 ? 0 : pxa + 1;
+    uxb = pxb;
+    uya = pya == 0 ? 0 : pya + 1;
+    uyb = pyb;
+    {
+      const int rw = uxb - uxa, cnt = rw * (uyb - uya);
+      const float rr = 1.0f / float(max(rw, 1));
+      for (int e0 = threadIdx.x; e0 < cnt; e0 += NT * B) {
+        int l[B];
+        P div[B], uo[B], ff[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+          const int e = e0 + j * NT;
+          if (e < cnt) {
+            const int ry = qdiv(e, rw, rr), x = uxa + (e - ry * rw), y = uya + ry;
+            l[j] = (y - ya) * LW + (x - xa);
+            const P cx = sX[l[j]], mx = sX[x > 0 ? l[j] - 1 : l[j]];
+            const P dx = (x < W - 1) ? cx - mx : -mx;
+            const P cy = sY[l[j]], my = sY[y > 0 ? l[j] - LW : l[j]];
+            const P dy = (y < H - 1) ? cy - my : -my;
+            div[j] = (x > 0 ? dx : cx) + (y > 0 ? dy : cy);
+            uo[j] = sU[l[j]];
+            ff[j] = sF[l[j]];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+          if (e0 + j * NT < cnt) {
+            P un = uo[j] + tau * (div[j] + lambda * ff[j]);
+            un = un * rcp(R(1) + tau * lambda);
+            sU[l[j]] = un;
+            sB[l[j]] = (R(1) + theta) * un - theta * uo[j];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const auto ou = gptr<P>(dir ? d.u : d.u2);
+  const auto ob = gptr<P>(dir ? d.ub : d.ub2);
+  const auto op = gptr<P>(dir ? d.p : d.p2);
+  const int iw = x1 - x0, icnt = iw * (y1 - y0);
+  const float riw = 1.0f / float(iw);
+  for (int e = threadIdx.x; e < icnt; e += NT) {
+    const int ry = qdiv(e, iw, riw), x = x0 + (e - ry * iw), y = y0 + ry;
+    const int l = (y - ya) * LW + (x - xa);
+    const size_t gi = size_t(y) * W + x;
+    ou[gi] = sU[l];
+    ob[gi] = sB[l];
+    op[gi] = sX[l];
+    op[gi + n] = sY[l];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += k;
+}
+
 // TGV denoising primal half (tgvoptimizer.cu:119-127): TV_primal_u_kernel
 // then TGV_primal_v_kernel at every pixel.
 template <class P, class S, int NT>
@@ -1863,6 +2031,37 @@ static void with_fft_len(int n, Fn&& fn) {
   fn(std::integral_constant<int, 0>{});
 }
 
+size_t tv_block_lds(int dtype, int tx, int ty, int k) {
+  const size_t e = dtype == DT_F32 ? 4 : (dtype == DT_C128 ? 16 : 8);
+  return size_t(5) * size_t(tx + 2 * k) * size_t(ty + 2 * k) * e;
+}
+
+// Workgroup size of a blocked ROF launch: about one batch of pixels per
+// thread of the loaded tile, 256 to 1024 threads.
+static int tv_block_threads(int tx, int ty, int k) {
+  const int px = (tx + 2 * k) * (ty + 2 * k);
+  return px > 2048 * kTvBlockBatch / 2 ? 1024 : px > 256 * kTvBlockBatch ? 512 : 256;
+}
+
+void run_tv_block(const Launch& L, int dtype, int dir) {
+  const int tx = L.tb_x, ty = L.tb_y, k = L.tb_k;
+  const size_t lds = tv_block_lds(dtype, tx, ty, k);
+  const int grid = ((L.tb_w + tx - 1) / tx) * ((L.tb_h + ty - 1) / ty);
+  auto launch = [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    switch (dtype) {
+      case DT_F32: hipLaunchKernelGGL((k_tv_block<float, NT>), dim3(grid), dim3(NT), lds, L.stream, L.sl, tx, ty, k, dir); break;
+      case DT_C64: hipLaunchKernelGGL((k_tv_block<Cx<float>, NT>), dim3(grid), dim3(NT), lds, L.stream, L.sl, tx, ty, k, dir); break;
+      case DT_F64: hipLaunchKernelGGL((k_tv_block<double, NT>), dim3(grid), dim3(NT), lds, L.stream, L.sl, tx, ty, k, dir); break;
+      case DT_C128: hipLaunchKernelGGL((k_tv_block<Cx<double>, NT>), dim3(grid), dim3(NT), lds, L.stream, L.sl, tx, ty, k, dir); break;
+    }
+  };
+  const int nt = tv_block_threads(tx, ty, k);
+  if (nt == 1024) launch(std::integral_constant<int, 1024>{});
+  else if (nt == 512) launch(std::integral_constant<int, 512>{});
+  else launch(std::integral_constant<int, 256>{});
+}
+
 bool fft_len_compiled(int dtype, int n) {
   if (dtype != DT_C64 && dtype != DT_F32) return false;
   for (int m : kFftLengths)
@@ -2016,6 +2215,12 @@ static bool set_lds_attrs(int b) {
     set((const void*)&k_cols<R, CM_FWD, false, NC, NF>);
     set((const void*)&k_cols<R, CM_ADJ, false, NC, NF>);
   };
+  set((const void*)&k_tv_block<R, 256>);
+  set((const void*)&k_tv_block<Cx<R>, 256>);
+  set((const void*)&k_tv_block<R, 512>);
+  set((const void*)&k_tv_block<Cx<R>, 512>);
+  set((const void*)&k_tv_block<R, 1024>);
+  set((const void*)&k_tv_block<Cx<R>, 1024>);
   with_fft_len<R>(0, one);
   for (int n : kFftLengths)
     if (std::is_same<R, float>::value) with_fft_len<R>(n, one);

@@ -29,8 +29,9 @@ bool g_loaded = false;
 const char* cache_path() { return std::getenv("PDT_TUNE_CACHE"); }
 
 // File format: one entry per line,
-// "kind dt W H Hk C nc ns : cb cg rpi [cb_a [wpc0 wpc1 wpc2 wpc3]]"
-// (cb_a absent: the adjoint row pass uses cb; occupancy caps absent: none).
+// "kind dt W H Hk C nc ns : cb cg rpi [cb_a [wpc0 wpc1 wpc2 wpc3 [tbx tby tbk]]]"
+// (cb_a absent: the adjoint row pass uses cb; occupancy caps absent: none;
+// blocked ROF tiles absent: the default).
 void load_locked() {
   if (g_loaded) return;
   g_loaded = true;
@@ -47,8 +48,12 @@ void load_locked() {
         colon == ':') {
       if (!(ss >> v.cb_a)) v.cb_a = 0;
       int w[4];
-      if (ss >> w[0] >> w[1] >> w[2] >> w[3])
+      if (ss >> w[0] >> w[1] >> w[2] >> w[3]) {
         for (int i = 0; i < 4; ++i) v.wpc[i] = w[i];
+        int t[3];
+        if (ss >> t[0] >> t[1] >> t[2])
+          for (int i = 0; i < 3; ++i) v.tb[i] = t[i];
+      }
       g_cache[k] = v;
     }
   }
@@ -61,7 +66,7 @@ void save_locked() {
   for (const auto& kv : g_cache)
     out << kv.first.str() << " : " << kv.second.cb << ' ' << kv.second.cg << ' ' << kv.second.rpi << ' '
         << kv.second.cb_a << ' ' << kv.second.wpc[0] << ' ' << kv.second.wpc[1] << ' ' << kv.second.wpc[2] << ' '
-        << kv.second.wpc[3] << '\n';
+        << kv.second.wpc[3] << ' ' << kv.second.tb[0] << ' ' << kv.second.tb[1] << ' ' << kv.second.tb[2] << '\n';
 }
 }  // namespace
 

@@ -23,6 +23,7 @@ struct TuneKey {
 struct TuneVal {
   int cb = 0, cg = 0, rpi = 0, cb_a = 0;
   int wpc[4] = {0, 0, 0, 0};  // workgroups per CU of the passes (0: no cap)
+  int tb[3] = {0, 0, -1};     // temporally blocked ROF tiles x, y, iterations (-1: default, 0: off)
 };
 
 TuneVal tune_lookup(const TuneKey& k);

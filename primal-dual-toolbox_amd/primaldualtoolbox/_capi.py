@@ -48,6 +48,11 @@ class PdtStats(ct.Structure):
                 ("ms", ct.c_double), ("converged", ct.c_int32), ("ms_to_stop", ct.c_double)]
 
 
+class PdtTiles(ct.Structure):
+    _fields_ = [("cb", ct.c_int32), ("cg", ct.c_int32), ("rpi", ct.c_int32), ("cb_a", ct.c_int32),
+                ("wpc", ct.c_int32 * 4), ("tb_x", ct.c_int32), ("tb_y", ct.c_int32), ("tb_k", ct.c_int32)]
+
+
 class PdtKernelProf(ct.Structure):
     _fields_ = [("name", ct.c_char * 32), ("ms", ct.c_double), ("bytes", ct.c_double)]
 
@@ -112,8 +117,8 @@ def lib():
         L.pdt_batch_profile.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ct.POINTER(PdtKernelProf), ct.c_int,
                                          ct.POINTER(ct.c_int)]
         ip = ct.POINTER(ct.c_int)
-        L.pdt_autotune.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ip, ip, ip, ip, ip, dp]
-        L.pdt_set_tiles.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ip]
+        L.pdt_autotune.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ct.POINTER(PdtTiles), dp]
+        L.pdt_set_tiles.argtypes = [ct.POINTER(vp), ct.c_int, ct.POINTER(PdtTiles)]
         L.pdt_batch_solve.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int]
         fp, szp = ct.POINTER(ct.c_float), ct.POINTER(ct.c_size_t)
         L.pdt_cnn_create.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(vp)]

@@ -250,33 +250,40 @@ class _Optimizer:
             self._h = None
 
 
+def _tiles_dict(t) -> dict:
+    return {"cb": t.cb, "cg": t.cg, "rpi": t.rpi, "cb_a": t.cb_a, "wg_per_cu": list(t.wpc),
+            "tv_block": [t.tb_x, t.tb_y, t.tb_k]}
+
+
 def autotune(optimizers, reps: int = 20) -> dict:
-    """Times the candidate tile shapes, then occupancy caps (workgroups per
-    CU) per pass, on these configured optimizers (one batch), caches the
-    fastest and returns it.  Leaves the state advanced: reset with
-    setInput0 / setNoisyData before solving."""
+    """Times the candidate tile shapes (for one ROF slice also temporally
+    blocked launches), then occupancy caps (workgroups per CU) per pass, on
+    these configured optimizers (one batch), caches the fastest and returns
+    it.  Leaves the state advanced: reset with setInput0 / setNoisyData
+    before solving."""
     for o in optimizers:
         o._push_params()
     arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
-    cb, cg, rpi, cb_a, ms = ct.c_int(), ct.c_int(), ct.c_int(), ct.c_int(), ct.c_double()
-    wpc = (ct.c_int * 4)()
-    C.check(C.lib().pdt_autotune(arr, len(optimizers), int(reps), ct.byref(cb), ct.byref(cg), ct.byref(rpi),
-                                 ct.byref(cb_a), wpc, ct.byref(ms)))
-    return {"cb": cb.value, "cg": cg.value, "rpi": rpi.value, "cb_a": cb_a.value, "wg_per_cu": list(wpc),
-            "ms_per_iteration": ms.value}
+    t, ms = C.PdtTiles(), ct.c_double()
+    C.check(C.lib().pdt_autotune(arr, len(optimizers), int(reps), ct.byref(t), ct.byref(ms)))
+    return dict(_tiles_dict(t), ms_per_iteration=ms.value)
 
 
-def set_tiles(optimizers, cb: int = 0, cg: int = 0, rpi: int = 0, cb_a: int = 0, wg_per_cu=None) -> None:
+def set_tiles(optimizers, cb: int = 0, cg: int = 0, rpi: int = 0, cb_a: int = 0, wg_per_cu=None,
+              tv_block=None) -> None:
     """Stores a manual tile choice for the shape of these optimizers (one
     batch) in the tile cache the autotuner uses: coils per forward and
     adjoint row item (cb, cb_a), columns per column item (cg), rows per
     denoising item (rpi), 0 = the default; wg_per_cu: 4 occupancy caps
-    (forward row, column, adjoint row, denoising passes; 0 = none)."""
+    (forward row, column, adjoint row, denoising passes; 0 = none);
+    tv_block: (tile x, tile y, iterations per launch) of the temporally
+    blocked ROF launches, None or iterations 0 = off."""
     for o in optimizers:
         o._push_params()
     arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
-    wpc = None if wg_per_cu is None else (ct.c_int * 4)(*[int(x) for x in wg_per_cu])
-    C.check(C.lib().pdt_set_tiles(arr, len(optimizers), int(cb), int(cg), int(rpi), int(cb_a), wpc))
+    t = C.PdtTiles(int(cb), int(cg), int(rpi), int(cb_a), (ct.c_int32 * 4)(*(wg_per_cu or (0, 0, 0, 0))),
+                   *[int(v) for v in (tv_block or (0, 0, 0))])
+    C.check(C.lib().pdt_set_tiles(arr, len(optimizers), ct.byref(t)))
 
 
 def batch_profile(optimizers, reps: int = 20) -> list:

@@ -189,17 +189,16 @@ def test_batch_checks_fire_before_any_launch():
     d = _solver(C.PDT_SOLVER_TGV, C.PDT_C64, 0)
     try:
         arr = lambda *hs: (ct.c_void_p * len(hs))(*hs)  # noqa: E731
-        out = [ct.c_int() for _ in range(4)]
         ms = ct.c_double()
         cases = [(arr(a, b), "share dual_fp16"), (arr(a, c), "share kind and dtype"),
                  (arr(a, d, a), "appears twice")]
         for hs, msg in cases:
             assert L.pdt_batch_solve(hs, len(hs), 0) == C.PDT_ERR_ARG
             assert msg.encode() in L.pdt_last_error()
-            wpc = (ct.c_int * 4)()
-            assert L.pdt_autotune(hs, len(hs), 1, *[ct.byref(o) for o in out], wpc, ct.byref(ms)) == C.PDT_ERR_ARG
+            tiles = C.PdtTiles(1, 1, 1, 1)
+            assert L.pdt_autotune(hs, len(hs), 1, ct.byref(tiles), ct.byref(ms)) == C.PDT_ERR_ARG
             assert msg.encode() in L.pdt_last_error()
-            assert L.pdt_set_tiles(hs, len(hs), 1, 1, 1, 1, None) == C.PDT_ERR_ARG
+            assert L.pdt_set_tiles(hs, len(hs), ct.byref(tiles)) == C.PDT_ERR_ARG
             assert msg.encode() in L.pdt_last_error()
         # a consistent batch gets past the checks to the reference's state error
         assert L.pdt_batch_solve(arr(a, d), 2, 0) == C.PDT_ERR_STATE
