@@ -223,6 +223,8 @@ typedef struct pdt_tiles {
   int32_t tb_x, tb_y, tb_k; /* ROF, one slice: tb_k iterations per launch on
                       tb_x x tb_y tiles held in LDS (tb_k 0: one iteration per
                       pair of launches)                                          */
+  int32_t dual_px; /* TGV denoising: pixels per thread of the dual step, 1 or 2
+                      (2 needs even widths; 0: the default, 2 where possible)   */
 } pdt_tiles;
 
 /* NEW: autotuner.  Times every candidate tile shape (the fields of

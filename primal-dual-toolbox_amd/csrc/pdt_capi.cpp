@@ -791,6 +791,8 @@ void prepare_op_data(pdt_solver* s) {
 struct Engine {
   int kind = 0, dtype = 0;
   bool fp16 = false;
+  bool pair_duals = true;  // TGV denoising: the dual step as 2-pixel vectors (even widths)
+  int force_dpx = 0;       // dual step pixels per thread (0: cache / default)
   int force_cb = 0, force_cg = 0, force_rpi = 0, force_cb_a = 0;  // autotuner overrides (0: cache / default)
   int force_wpc[4] = {-1, -1, -1, -1};  // occupancy caps (-1: cache; 0: none)
   int force_tb[3] = {0, 0, -1};         // blocked ROF tiles (tb[2] -1: cache / default; 0: off)
@@ -876,6 +878,8 @@ struct Engine {
       TuneVal tv = tune_lookup(key);
       for (int i = 0; i < 4; ++i) L.wpc[i] = force_wpc[i] >= 0 ? force_wpc[i] : tv.wpc[i];
       rpi = force_rpi > 0 ? force_rpi : tv.rpi > 0 ? tv.rpi : 1;
+      const int dpx = force_dpx > 0 ? force_dpx : tv.dual_px > 0 ? tv.dual_px : 2;
+      pair_duals = dpx == 2;
       // temporally blocked ROF launches: one slice only, tiles that fit the LDS
       const int* tb = force_tb[2] >= 0 ? force_tb : tv.tb[2] >= 0 ? tv.tb : kDefaultTvBlock;
       if (kind == PDT_SOLVER_TV && ns == 1 && tb[2] > 0 && tb[0] > 0 && tb[1] > 0 &&
@@ -984,8 +988,9 @@ struct Engine {
     // signature of everything the kernels and graphs depend on (the control
     // and statistics words are owned by the engine and added below)
     sig.assign(reinterpret_cast<const char*>(hs.data()), hs.size() * sizeof(SliceDev));
-    const int tiles[12] = {cb,       cg,       rpi,      int(sv[0]->prm.gap_every), cb_a,   L.wpc[0],
-                           L.wpc[1], L.wpc[2], L.wpc[3], L.tb_x,                    L.tb_y, L.tb_k};
+    const int tiles[13] = {cb,       cg,       rpi,      int(sv[0]->prm.gap_every), cb_a,   L.wpc[0],
+                           L.wpc[1], L.wpc[2], L.wpc[3], L.tb_x,                    L.tb_y, L.tb_k,
+                           int(pair_duals)};
     sig.append(reinterpret_cast<const char*>(tiles), sizeof(tiles));
     if (plan_only) return;
     cs_bytes = size_t(ns) * (ST_WORDS * sizeof(double) + CTL_WORDS * sizeof(int));
@@ -1153,6 +1158,10 @@ struct Engine {
         for (int i : groups[r.g]) maxW = std::max(maxW, hs[i].W);
         const int px = maxW * L.rpi;
         x.den_threads = px > 512 ? 1024 : px > 256 ? 512 : kBlock;
+        // the TGV dual step as 2-pixel vectors where every width is even
+        bool even = true;
+        for (int i : groups[r.g]) even = even && hs[i].W % 2 == 0;
+        x.den_pair = kind == PDT_SOLVER_TGV && even && pair_duals;
       }
       x.dense = dense;
       x.ndual = ndual;
@@ -2118,6 +2127,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
     struct Cand {
       std::array<int, 4> t;
       std::array<int, 3> tb;
+      int dpx = 0;
     };
     std::vector<Cand> cand;
     if (mri) {
@@ -2129,16 +2139,18 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
       for (int cb : cbs)
         if (cb <= maxC)
           for (int cg : {1, 2, 4}) {
-            cand.push_back({{cb, cg, 0, cb}, {0, 0, 0}});
-            if (cb != maxC) cand.push_back({{cb, cg, 0, maxC}, {0, 0, 0}});
+            cand.push_back({{cb, cg, 0, cb}, {0, 0, 0}, 0});
+            if (cb != maxC) cand.push_back({{cb, cg, 0, maxC}, {0, 0, 0}, 0});
           }
     } else {
-      for (int rpi : {1, 2, 4, 8}) cand.push_back({{0, 0, rpi, 0}, {0, 0, 0}});
+      for (int rpi : {1, 2, 4, 8})
+        for (int dpx : sv[0]->kind == PDT_SOLVER_TGV ? std::vector<int>{1, 2} : std::vector<int>{0})
+          cand.push_back({{0, 0, rpi, 0}, {0, 0, 0}, dpx});
       if (sv[0]->kind == PDT_SOLVER_TV && sv.size() == 1)
         for (const auto& tb : std::vector<std::array<int, 3>>{
                  {16, 16, 4}, {16, 16, 6}, {32, 16, 4}, {32, 16, 8}, {32, 32, 8}, {64, 32, 8}, {32, 32, 12},
                  {64, 64, 8}, {64, 64, 16}})
-          cand.push_back({{0, 0, 1, 0}, tb});
+          cand.push_back({{0, 0, 1, 0}, tb, 0});
     }
     auto engine = [&](const Cand& c, const int (&wpc)[4]) {
       auto e = std::make_unique<Engine>();
@@ -2148,6 +2160,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
       e->force_rpi = c.t[2];
       e->force_cb_a = c.t[3];
       for (int i = 0; i < 3; ++i) e->force_tb[i] = c.tb[i];
+      e->force_dpx = c.dpx;
       for (int i = 0; i < 4; ++i) e->force_wpc[i] = wpc[i];
       e->build();
       e->reset_ctl();
@@ -2155,7 +2168,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
     };
     const int none[4] = {0, 0, 0, 0};
     double best = 1e300;
-    Cand bc{{0, 0, 0, 0}, {0, 0, 0}};
+    Cand bc{{0, 0, 0, 0}, {0, 0, 0}, 0};
     std::vector<double> best_st;
     TuneKey key{};
     for (const auto& c : cand) {
@@ -2167,7 +2180,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
       key = e->key;
       if (ms < best) {
         best = ms;
-        bc = {{e->L.cb, e->L.cg, e->L.rpi, e->L.cb_a}, {e->G[0].tb_x, e->G[0].tb_y, e->G[0].tb_k}};
+        bc = {{e->L.cb, e->L.cg, e->L.rpi, e->L.cb_a}, {e->G[0].tb_x, e->G[0].tb_y, e->G[0].tb_k}, c.dpx};
         best_st = st;
       }
     }
@@ -2203,6 +2216,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
     TuneVal tv = mri ? TuneVal{bc.t[0], bc.t[1], 0, bc.t[3]} : TuneVal{0, 0, bc.t[2], 0};
     for (int i = 0; i < 4; ++i) tv.wpc[i] = wpc[i];
     for (int i = 0; i < 3; ++i) tv.tb[i] = bc.tb[i];
+    tv.dual_px = bc.dpx;
     tune_store(key, tv);
     double ms = 0;
     {
@@ -2218,6 +2232,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
       out->tb_x = bc.tb[0];
       out->tb_y = bc.tb[1];
       out->tb_k = bc.tb[2];
+      out->dual_px = bc.dpx;
     }
     if (out_ms) *out_ms = ms;
   });
@@ -2241,6 +2256,7 @@ int pdt_set_tiles(pdt_solver** solvers, int count, const pdt_tiles* t) {
     e.force_tb[0] = t->tb_x;
     e.force_tb[1] = t->tb_y;
     e.force_tb[2] = t->tb_k > 0 ? t->tb_k : 0;
+    e.force_dpx = t->dual_px == 1 || t->dual_px == 2 ? t->dual_px : 0;
     e.build();
     const bool mri = sv[0]->is_op();
     TuneVal tv = mri ? TuneVal{e.L.cb, e.L.cg, 0, e.L.cb_a} : TuneVal{0, 0, e.L.rpi, 0};
@@ -2248,6 +2264,7 @@ int pdt_set_tiles(pdt_solver** solvers, int count, const pdt_tiles* t) {
     tv.tb[0] = e.L.tb_x;
     tv.tb[1] = e.L.tb_y;
     tv.tb[2] = e.L.tb_k;
+    tv.dual_px = e.force_dpx;
     tune_store(e.key, tv);
   });
 }

@@ -207,24 +207,40 @@ __global__ __launch_bounds__(kConvThreads) void wgrad_kernel(const __bf16* __res
   if (wave == 0 && lane < COUT) pb[size_t(MTN) * 16 * KP + lane] = db;
 }
 
-// Sums the per-block partials in block order into the master-layout
-// gradient: dW[tap][co][ci] and db[co].
-__global__ __launch_bounds__(256) void wgrad_reduce(const float* part, int nblk, int cin, int cout, float* g) {
+// Sums the per-block partials into the master-layout gradient dW[tap][co][ci]
+// and db[co].  A workgroup takes kRedOut consecutive outputs; its kRedGroups
+// thread groups each sum every kRedGroups-th block partial, in block order
+// (a group reads kRedOut neighbouring floats of a partial row at a time),
+// and the group sums are added in group order: a fixed summation order, so
+// the gradient is reproducible run to run.
+constexpr int kRedOut = 64, kRedGroups = 16;
+
+__global__ __launch_bounds__(kRedOut * kRedGroups) void wgrad_reduce(const float* part, int nblk, int cin, int cout,
+                                                                     float* g) {
+  __shared__ double red[kRedGroups][kRedOut];
   const int KP = kpad(cin), MT = (cout + 15) / 16 * 16;
   const int nw = 9 * cout * cin;
   const size_t stride = size_t(MT) * KP + cout;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nw + cout) return;
-  size_t src;
-  if (e < nw) {
-    const int tap = e / (cout * cin), r = e - tap * cout * cin, co = r / cin, ci = r - co * cin;
-    src = size_t(co) * KP + tap * cin + ci;
-  } else {
-    src = size_t(MT) * KP + (e - nw);
-  }
+  const int o = threadIdx.x % kRedOut, grp = threadIdx.x / kRedOut;
+  const int e = blockIdx.x * kRedOut + o;
   double s = 0;
-  for (int b = 0; b < nblk; ++b) s += part[size_t(b) * stride + src];
-  g[e] = float(s);
+  if (e < nw + cout) {
+    size_t src;
+    if (e < nw) {
+      const int tap = e / (cout * cin), r = e - tap * cout * cin, co = r / cin, ci = r - co * cin;
+      src = size_t(co) * KP + tap * cin + ci;
+    } else {
+      src = size_t(MT) * KP + (e - nw);
+    }
+    for (int b = grp; b < nblk; b += kRedGroups) s += part[size_t(b) * stride + src];
+  }
+  red[grp][o] = s;
+  __syncthreads();
+  if (grp == 0 && e < nw + cout) {
+    double t = 0;
+    for (int j = 0; j < kRedGroups; ++j) t += red[j][o];
+    g[e] = float(t);
+  }
 }
 
 // Master fp32 W[tap][co][ci] -> bf16 Wk[co][tap * cin + ci] (padded) and
@@ -375,7 +391,8 @@ void cnn_conv_wgrad(hipStream_t st, int cin, int cout, const void* in, const voi
   else if (cin == 32 && cout == 6)
     hipLaunchKernelGGL((wgrad_kernel<32, 6>), dim3(nblk), dim3(kConvThreads), 0, st, x, z, part, n, h, w);
   const int ne = 9 * cin * cout + cout;
-  hipLaunchKernelGGL(wgrad_reduce, dim3((ne + 255) / 256), dim3(256), 0, st, part, nblk, cin, cout, grad);
+  hipLaunchKernelGGL(wgrad_reduce, dim3((ne + kRedOut - 1) / kRedOut), dim3(kRedOut * kRedGroups), 0, st, part, nblk,
+                     cin, cout, grad);
 }
 
 size_t cnn_wk_elems(int cin, int cout) { return size_t((cout + 15) / 16 * 16) * ((9 * cin + 31) / 32 * 32); }

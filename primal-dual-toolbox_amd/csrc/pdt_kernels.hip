@@ -114,9 +114,19 @@ template <class P, class S> struct Ld {
   template <class A> static __device__ __forceinline__ P get(A* a, size_t i) { return a[i]; }
   template <class A> static __device__ __forceinline__ void put(A* a, size_t i, P v) { a[i] = v; }
 };
+// The value stored is the float result rounded to nearest half: the empty
+// register constraint keeps the compiler from fusing the product that
+// produced it with the conversion (a mixed-precision FMA rounds once, straight
+// to half), so every kernel stores the same halves.
+__device__ __forceinline__ _Float16 to_half(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+v"(v));
+#endif
+  return _Float16(v);
+}
 template <> struct Ld<float, _Float16> {
   template <class A> static __device__ __forceinline__ float get(A* a, size_t i) { return float(a[i]); }
-  template <class A> static __device__ __forceinline__ void put(A* a, size_t i, float v) { a[i] = _Float16(v); }
+  template <class A> static __device__ __forceinline__ void put(A* a, size_t i, float v) { a[i] = to_half(v); }
 };
 template <> struct Ld<Cx<float>, Cx<_Float16>> {
   template <class A> static __device__ __forceinline__ Cx<float> get(A* a, size_t i) {
@@ -124,7 +134,7 @@ template <> struct Ld<Cx<float>, Cx<_Float16>> {
     return {float(h.re), float(h.im)};
   }
   template <class A> static __device__ __forceinline__ void put(A* a, size_t i, Cx<float> v) {
-    a[i] = Cx<_Float16>{_Float16(v.re), _Float16(v.im)};
+    a[i] = Cx<_Float16>{to_half(v.re), to_half(v.im)};
   }
 };
 
@@ -694,15 +704,56 @@ __device__ __forceinline__ R sched_alpha(const SliceDev& d, double a_start, doub
   return exp(dk / dmax * log(R(a_end)) + (dmax - dk) / dmax * log(R(a_start)));
 }
 
-// Dual ascent + projection of p (and q) at one pixel, with optional gap terms
-// t[0] += |grad ub - vb|, t[1] += |E vb|_F, t[2] += Re<grad ub - vb, p> +
-// Re<E vb, q>_F (TGV_dual_p_kernel / TGV_dual_q_kernel / TV_dual_p_kernel).
-// ia0, ia1 = 1 / alpha0, 1 / alpha1 (TGV only).
+// The arithmetic of the dual step at one pixel, on values: c = ubar, uxp /
+// uyp = ubar at the forward neighbours (c itself at the last column / row);
+// TGV: vx0, vy0 = vbar, vxx = vbar_x and vyx = vbar_y at the x neighbour,
+// vxy, vyy at the y neighbour.  Updates p (and q) in place, with optional
+// gap terms t[0] += |grad ub - vb|, t[1] += |E vb|_F, t[2] += Re<grad ub -
+// vb, p> + Re<E vb, q>_F (TGV_dual_p_kernel / TGV_dual_q_kernel /
+// TV_dual_p_kernel).  ia0, ia1 = 1 / alpha0, 1 / alpha1 (TGV only).
+template <class P, bool TGV, bool GAP>
+__device__ __forceinline__ void dual_math(P c, P uxp, P uyp, P vx0, P vy0, P vxx, P vyx, P vxy, P vyy, P& px, P& py,
+                                          P& qxx, P& qyy, P& qxy, typename RealOf<P>::T sigma,
+                                          typename RealOf<P>::T ia0, typename RealOf<P>::T ia1, double* t) {
+  using R = typename RealOf<P>::T;
+  P tx = uxp - c, ty = uyp - c;
+  if (TGV) {
+    tx = tx - vx0;
+    ty = ty - vy0;
+    const P exx = vxx - vx0;
+    const P eyy = vyy - vy0;
+    const P exy = R(0.5) * (vxy - vx0 + vyx - vy0);
+    const P bxx = qxx + sigma * exx;
+    const P byy = qyy + sigma * eyy;
+    const P bxy = qxy + sigma * exy;
+    const R qn = sqrt(abs2(bxx) + abs2(byy) + R(2) * abs2(bxy));
+    const R sq = rcp(fmax(R(1), qn * ia0));
+    qxx = bxx * sq;
+    qyy = byy * sq;
+    qxy = bxy * sq;
+    if (GAP) {
+      t[1] += sqrt(dabs2(exx) + dabs2(eyy) + 2.0 * dabs2(exy));
+      t[2] += redot(exx, qxx) + redot(eyy, qyy) + 2.0 * redot(exy, qxy);
+    }
+  }
+  const P ax = px + sigma * tx;
+  const P ay = py + sigma * ty;
+  const R pn = sqrt(abs2(ax) + abs2(ay));
+  const R sp = rcp(fmax(R(1), TGV ? pn * ia1 : pn));
+  px = ax * sp;
+  py = ay * sp;
+  if (GAP) {
+    t[0] += sqrt(dabs2(tx) + dabs2(ty));
+    t[2] += redot(tx, px) + redot(ty, py);
+  }
+}
+
+// Dual ascent + projection of p (and q) at one pixel (dual_math on the
+// pixel's values in memory).
 template <class P, class S, bool TGV, bool GAP>
 __device__ __forceinline__ void dual_pixel(const SliceDev& d, int x, int y, typename RealOf<P>::T sigma,
                                            typename RealOf<P>::T ia0, typename RealOf<P>::T ia1,
                                            double* t) {
-  using R = typename RealOf<P>::T;
   using L = Ld<P, S>;
   const int W = d.W, H = d.H;
   const size_t n = size_t(W) * H;
@@ -711,45 +762,34 @@ __device__ __forceinline__ void dual_pixel(const SliceDev& d, int x, int y, type
   const auto ub = gptr<const P>(d.ub);
   const auto px = gptr<S>(d.p);
   const auto py = px + n;
-  const P c = ub[i];
-  P tx = ub[ixp] - c, ty = ub[iyp] - c;
+  P vx0 = pzero<P>(), vy0 = vx0, vxx = vx0, vyx = vx0, vxy = vx0, vyy = vx0;
+  P qxx = vx0, qyy = vx0, qxy = vx0;
+  const auto qxxp = gptr<S>(d.q);
+  const auto qyyp = qxxp + n;
+  const auto qxyp = qyyp + n;
   if (TGV) {
     const auto vbx = gptr<const P>(d.vb);
     const auto vby = vbx + n;
-    const P vx0 = vbx[i], vy0 = vby[i];
-    tx = tx - vx0;
-    ty = ty - vy0;
-    const P exx = vbx[ixp] - vx0;
-    const P eyy = vby[iyp] - vy0;
-    const P exy = R(0.5) * (vbx[iyp] - vx0 + vby[ixp] - vy0);
-    const auto qxx = gptr<S>(d.q);
-    const auto qyy = qxx + n;
-    const auto qxy = qyy + n;
-    const P bxx = L::get(qxx, i) + sigma * exx;
-    const P byy = L::get(qyy, i) + sigma * eyy;
-    const P bxy = L::get(qxy, i) + sigma * exy;
-    const R qn = sqrt(abs2(bxx) + abs2(byy) + R(2) * abs2(bxy));
-    const R sq = rcp(fmax(R(1), qn * ia0));
-    const P nxx = bxx * sq, nyy = byy * sq, nxy = bxy * sq;
-    L::put(qxx, i, nxx);
-    L::put(qyy, i, nyy);
-    L::put(qxy, i, nxy);
-    if (GAP) {
-      t[1] += sqrt(dabs2(exx) + dabs2(eyy) + 2.0 * dabs2(exy));
-      t[2] += redot(exx, nxx) + redot(eyy, nyy) + 2.0 * redot(exy, nxy);
-    }
+    vx0 = vbx[i];
+    vy0 = vby[i];
+    vxx = vbx[ixp];
+    vyx = vby[ixp];
+    vxy = vbx[iyp];
+    vyy = vby[iyp];
+    qxx = L::get(qxxp, i);
+    qyy = L::get(qyyp, i);
+    qxy = L::get(qxyp, i);
   }
-  const P ax = L::get(px, i) + sigma * tx;
-  const P ay = L::get(py, i) + sigma * ty;
-  const R pn = sqrt(abs2(ax) + abs2(ay));
-  const R sp = rcp(fmax(R(1), TGV ? pn * ia1 : pn));
-  const P nx = ax * sp, ny = ay * sp;
-  L::put(px, i, nx);
-  L::put(py, i, ny);
-  if (GAP) {
-    t[0] += sqrt(dabs2(tx) + dabs2(ty));
-    t[2] += redot(tx, nx) + redot(ty, ny);
+  P pxv = L::get(px, i), pyv = L::get(py, i);
+  dual_math<P, TGV, GAP>(ub[i], ub[ixp], ub[iyp], vx0, vy0, vxx, vyx, vxy, vyy, pxv, pyv, qxx, qyy, qxy, sigma, ia0,
+                         ia1, t);
+  if (TGV) {
+    L::put(qxxp, i, qxx);
+    L::put(qyyp, i, qyy);
+    L::put(qxyp, i, qxy);
   }
+  L::put(px, i, pxv);
+  L::put(py, i, pyv);
 }
 
 // v-step of TGV_primal_v_kernel (tgv_kernels.cuh:647-686) at one pixel;
@@ -1091,6 +1131,83 @@ __global__ __launch_bounds__(NT) void k_tgv_dual(const SliceDev* sl, const int2*
     if (GAP) {
       const size_t i = size_t(y) * W + x;
       t[3] += dabs2(ub[i] - f[i]);
+    }
+  }
+  if (GAP) block_sum<4, NT>(reinterpret_cast<double(&)[4]>(t), part + size_t(bi) * kGapTerms);
+  if (ya == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += 1;
+}
+
+// Two pixels of storage type S side by side (one 8- or 16-byte access).
+template <class S> struct alignas(2 * sizeof(S)) Pair {
+  S a, b;
+};
+
+// k_tgv_dual with two horizontally adjacent pixels per thread (W even): the
+// duals, ubar and vbar move as 2-pixel vectors (8 bytes per p, q plane access
+// with fp16 storage, 16 with float2), the arithmetic is dual_math per pixel,
+// so the iterates equal k_tgv_dual's bit for bit.
+template <class P, class S, bool GAP, int NT>
+__global__ __launch_bounds__(NT) void k_tgv_dual2(const SliceDev* sl, const int2* items, int rpi, double* part) {
+  const int bi = xcd_item(blockIdx.x, gridDim.x);  // neighbouring rows on one XCD
+  const int2 it = items ? items[bi] : make_int2(0, bi * rpi);
+  const SliceDev& d = sl[it.x];
+  if (!slice_active(d)) return;
+  using R = typename RealOf<P>::T;
+  using L = Ld<P, S>;
+  const int W = d.W, H = d.H;
+  const size_t n = size_t(W) * H;
+  const int ya = it.y, yb = min(H, ya + rpi);
+  const R sigma = R(d.sigma), ia0 = rcp(R(d.alpha0)), ia1 = rcp(R(d.alpha1));
+  const auto ub = gptr<const P>(d.ub);
+  const auto ubp = gptr<const Pair<P>>(d.ub);
+  const auto vbx = gptr<const P>(d.vb);
+  const auto vbxp = gptr<const Pair<P>>(d.vb);
+  const auto f = gptr<const P>(d.f);
+  const auto pp = gptr<Pair<S>>(d.p);
+  const auto qp = gptr<Pair<S>>(d.q);
+  const size_t n2 = n / 2;  // pairs per plane
+  double t[kGapTerms] = {0};
+  const int W2 = W / 2, cnt = (yb - ya) * W2;
+  for (int idx = threadIdx.x; idx < cnt; idx += NT) {
+    const int yy = idx / W2;
+    const int x = 2 * (idx - yy * W2), y = ya + yy;
+    const size_t i = size_t(y) * W + x, i2 = i / 2;
+    const bool last = x + 2 >= W, bottom = y >= H - 1;
+    const size_t dn2 = bottom ? i2 : i2 + W2;  // pair below (the pair itself on the last row)
+    const Pair<P> u0 = ubp[i2], ud = ubp[dn2];
+    const P u2 = last ? u0.b : ub[i + 2];
+    const Pair<P> vx = vbxp[i2], vy = vbxp[n2 + i2], vxd = vbxp[dn2], vyd = vbxp[n2 + dn2];
+    const P vx2 = last ? vx.b : vbx[i + 2], vy2 = last ? vy.b : vbx[n + i + 2];
+    Pair<S> ps[2] = {pp[i2], pp[n2 + i2]};
+    Pair<S> qs[3] = {qp[i2], qp[n2 + i2], qp[2 * n2 + i2]};
+    P pxa = L::get(&ps[0].a, 0), pya = L::get(&ps[1].a, 0);
+    P qxxa = L::get(&qs[0].a, 0), qyya = L::get(&qs[1].a, 0), qxya = L::get(&qs[2].a, 0);
+    P pxb = L::get(&ps[0].b, 0), pyb = L::get(&ps[1].b, 0);
+    P qxxb = L::get(&qs[0].b, 0), qyyb = L::get(&qs[1].b, 0), qxyb = L::get(&qs[2].b, 0);
+    // pixel x (never the last column), then x + 1
+    dual_math<P, true, GAP>(u0.a, u0.b, bottom ? u0.a : ud.a, vx.a, vy.a, vx.b, vy.b, bottom ? vx.a : vxd.a,
+                            bottom ? vy.a : vyd.a, pxa, pya, qxxa, qyya, qxya, sigma, ia0, ia1, t);
+    dual_math<P, true, GAP>(u0.b, last ? u0.b : u2, bottom ? u0.b : ud.b, vx.b, vy.b, last ? vx.b : vx2,
+                            last ? vy.b : vy2, bottom ? vx.b : vxd.b, bottom ? vy.b : vyd.b, pxb, pyb, qxxb, qyyb,
+                            qxyb, sigma, ia0, ia1, t);
+    L::put(&ps[0].a, 0, pxa);
+    L::put(&ps[0].b, 0, pxb);
+    L::put(&ps[1].a, 0, pya);
+    L::put(&ps[1].b, 0, pyb);
+    L::put(&qs[0].a, 0, qxxa);
+    L::put(&qs[0].b, 0, qxxb);
+    L::put(&qs[1].a, 0, qyya);
+    L::put(&qs[1].b, 0, qyyb);
+    L::put(&qs[2].a, 0, qxya);
+    L::put(&qs[2].b, 0, qxyb);
+    pp[i2] = ps[0];
+    pp[n2 + i2] = ps[1];
+    qp[i2] = qs[0];
+    qp[n2 + i2] = qs[1];
+    qp[2 * n2 + i2] = qs[2];
+    if (GAP) {
+      t[3] += dabs2(u0.a - f[i]);
+      t[3] += dabs2(u0.b - f[i + 1]);
     }
   }
   if (GAP) block_sum<4, NT>(reinterpret_cast<double(&)[4]>(t), part + size_t(bi) * kGapTerms);
@@ -1989,6 +2106,11 @@ static void denoise_stage(const Launch& L, int kind, bool gap, int stage) {
       }
     } else if (stage == 0) {
       hipLaunchKernelGGL((k_tgv_primal<P, S, NT>), dim3(n), dim3(NT), lds, L.stream, L.sl, rows, L.rpi);
+    } else if (L.den_pair) {
+      // two pixels per thread: half the threads of the one-pixel launch (at least 256)
+      constexpr int NP = NT > kBlock ? NT / 2 : kBlock;
+      if (gap) hipLaunchKernelGGL((k_tgv_dual2<P, S, true, NP>), dim3(n), dim3(NP), lds, L.stream, L.sl, rows, L.rpi, L.part[0]);
+      else hipLaunchKernelGGL((k_tgv_dual2<P, S, false, NP>), dim3(n), dim3(NP), lds, L.stream, L.sl, rows, L.rpi, L.part[0]);
     } else if (gap) {
       hipLaunchKernelGGL((k_tgv_dual<P, S, true, NT>), dim3(n), dim3(NT), lds, L.stream, L.sl, rows, L.rpi, L.part[0]);
     } else {

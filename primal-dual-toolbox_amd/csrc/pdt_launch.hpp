@@ -51,6 +51,9 @@ struct Launch {
   // workgroup size of the denoising half-iterations (256, 512 or 1024): the
   // smallest that covers rpi rows of the widest slice in one pass, at most 1024
   int den_threads = 256;
+  // TGV denoising dual step with two pixels per thread (every slice of the
+  // launch has an even width)
+  bool den_pair = false;
   // occupancy caps, workgroups per CU (0: none) of the forward row pass, the
   // column pass, the adjoint row pass and the denoising half-iterations: a
   // launch asks for enough LDS that no more than that many workgroups fit

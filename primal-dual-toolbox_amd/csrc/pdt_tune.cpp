@@ -29,9 +29,9 @@ bool g_loaded = false;
 const char* cache_path() { return std::getenv("PDT_TUNE_CACHE"); }
 
 // File format: one entry per line,
-// "kind dt W H Hk C nc ns : cb cg rpi [cb_a [wpc0 wpc1 wpc2 wpc3 [tbx tby tbk]]]"
+// "kind dt W H Hk C nc ns : cb cg rpi [cb_a [wpc0 wpc1 wpc2 wpc3 [tbx tby tbk [dpx]]]]"
 // (cb_a absent: the adjoint row pass uses cb; occupancy caps absent: none;
-// blocked ROF tiles absent: the default).
+// blocked ROF tiles or dual pixels per thread absent: the default).
 void load_locked() {
   if (g_loaded) return;
   g_loaded = true;
@@ -51,8 +51,10 @@ void load_locked() {
       if (ss >> w[0] >> w[1] >> w[2] >> w[3]) {
         for (int i = 0; i < 4; ++i) v.wpc[i] = w[i];
         int t[3];
-        if (ss >> t[0] >> t[1] >> t[2])
+        if (ss >> t[0] >> t[1] >> t[2]) {
           for (int i = 0; i < 3; ++i) v.tb[i] = t[i];
+          if (!(ss >> v.dual_px)) v.dual_px = 0;
+        }
       }
       g_cache[k] = v;
     }
@@ -66,7 +68,7 @@ void save_locked() {
   for (const auto& kv : g_cache)
     out << kv.first.str() << " : " << kv.second.cb << ' ' << kv.second.cg << ' ' << kv.second.rpi << ' '
         << kv.second.cb_a << ' ' << kv.second.wpc[0] << ' ' << kv.second.wpc[1] << ' ' << kv.second.wpc[2] << ' '
-        << kv.second.wpc[3] << ' ' << kv.second.tb[0] << ' ' << kv.second.tb[1] << ' ' << kv.second.tb[2] << '\n';
+        << kv.second.wpc[3] << ' ' << kv.second.tb[0] << ' ' << kv.second.tb[1] << ' ' << kv.second.tb[2] << ' ' << kv.second.dual_px << '\n';
 }
 }  // namespace
 

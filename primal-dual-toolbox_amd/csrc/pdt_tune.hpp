@@ -24,6 +24,7 @@ struct TuneVal {
   int cb = 0, cg = 0, rpi = 0, cb_a = 0;
   int wpc[4] = {0, 0, 0, 0};  // workgroups per CU of the passes (0: no cap)
   int tb[3] = {0, 0, -1};     // temporally blocked ROF tiles x, y, iterations (-1: default, 0: off)
+  int dual_px = 0;            // TGV denoising dual step: pixels per thread (0: default)
 };
 
 TuneVal tune_lookup(const TuneKey& k);

@@ -252,7 +252,7 @@ class _Optimizer:
 
 def _tiles_dict(t) -> dict:
     return {"cb": t.cb, "cg": t.cg, "rpi": t.rpi, "cb_a": t.cb_a, "wg_per_cu": list(t.wpc),
-            "tv_block": [t.tb_x, t.tb_y, t.tb_k]}
+            "tv_block": [t.tb_x, t.tb_y, t.tb_k], "dual_px": t.dual_px}
 
 
 def autotune(optimizers, reps: int = 20) -> dict:
@@ -270,19 +270,20 @@ def autotune(optimizers, reps: int = 20) -> dict:
 
 
 def set_tiles(optimizers, cb: int = 0, cg: int = 0, rpi: int = 0, cb_a: int = 0, wg_per_cu=None,
-              tv_block=None) -> None:
+              tv_block=None, dual_px: int = 0) -> None:
     """Stores a manual tile choice for the shape of these optimizers (one
     batch) in the tile cache the autotuner uses: coils per forward and
     adjoint row item (cb, cb_a), columns per column item (cg), rows per
     denoising item (rpi), 0 = the default; wg_per_cu: 4 occupancy caps
     (forward row, column, adjoint row, denoising passes; 0 = none);
     tv_block: (tile x, tile y, iterations per launch) of the temporally
-    blocked ROF launches, None or iterations 0 = off."""
+    blocked ROF launches, None or iterations 0 = off; dual_px: pixels per
+    thread of the TGV denoising dual step (1, 2; 0 = the default)."""
     for o in optimizers:
         o._push_params()
     arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
     t = C.PdtTiles(int(cb), int(cg), int(rpi), int(cb_a), (ct.c_int32 * 4)(*(wg_per_cu or (0, 0, 0, 0))),
-                   *[int(v) for v in (tv_block or (0, 0, 0))])
+                   *[int(v) for v in (tv_block or (0, 0, 0))], int(dual_px))
     C.check(C.lib().pdt_set_tiles(arr, len(optimizers), ct.byref(t)))
 
 
