@@ -155,6 +155,45 @@ def warm_start_steps(opt, case, steps: int, warmup: int):
     return out
 
 
+def held_out_end_states(gold: str) -> list:
+    """The held-out slices that have FP64 oracle end states in the fixture
+    file (the network never saw them): cold and warm time-to-gap, iterations,
+    and rel. L2 of each GPU end state vs the oracle's u at its own stop."""
+    from primaldualtoolbox import synthetic as syn
+    from primaldualtoolbox import warmstart as ws
+    if not os.path.exists(ws.DEFAULT_WEIGHTS) or not os.path.exists(gold):
+        return []
+    net = ws.WarmStartCNN.load()
+    out = []
+    with np.load(gold) as z:
+        seeds = sorted(int(k[6:]) for k in z.files if k.startswith("iters_") and int(k[6:]) != 0)
+        for seed in seeds:
+            case = syn.mri320_case(seed)
+            op, opt = build_solver(case, float(z[f"alpha1_{seed}"]))
+            u0 = op.adjoint(case["kspace"])
+            rec = {"seed": seed, "oracle_iterations": int(z[f"iters_{seed}"])}
+            for start in ("cold", "warm"):
+                best = None
+                for _ in range(3):  # the first solve pays graph capture
+                    opt.setNoisyData(case["kspace"])
+                    t0 = time.perf_counter()
+                    if start == "warm":
+                        net.warm_start(opt)
+                    else:
+                        opt.setInput0(u0)
+                    opt.solve()
+                    dt = time.perf_counter() - t0
+                    best = dt if best is None else min(best, dt)
+                st = opt.getStats()
+                if not st["converged"]:
+                    raise RuntimeError(f"held-out seed {seed} ({start}) did not reach rel. gap {GAP_TOL}: {st}")
+                rec[start] = {"seconds": round(best, 5), "iterations": int(st["iters"]),
+                              "rel_l2_vs_fp64_oracle": _rel(opt.getResult(), z[f"u_{seed}"])}
+            log(f"[bench] held-out seed {seed}: {rec}")
+            out.append(rec)
+    return out
+
+
 def extra_cnn(reps: int = 5) -> dict:
     """Warm-start network kernels alone: inference on one 320 x 320 slice and
     one Adam training step at batch 16 (random data), with MFMA TFLOP/s."""
@@ -600,6 +639,7 @@ def main():
         with np.load(gold) as z:
             oracle_end = {"iterations": int(z["iters_0"]), "rel_l2_cold": _rel(u_cold, z["u_0"]),
                           "rel_l2_warm": _rel(warm["u"], z["u_0"]) if warm else None}
+    held_out = [] if args.no_warm else held_out_end_states(gold)
 
     # per-kernel durations (hipEvents on the solver stream) and algorithmic bytes
     prof = opt.profileKernels(args.profile_reps)
@@ -650,6 +690,7 @@ def main():
             "warm_vs_cold_rel_l2_u": _rel(warm["u"], u_cold) if warm else None,
             "iterations_to_gap": int(np.median(iters)),
             "fp64_oracle_end_state": oracle_end,
+            "held_out_vs_fp64_oracle": held_out,
             "final_rel_gap": float(np.median(gaps)),
             "device_loop_ms": float(np.median(loop_ms)),
             "us_per_iteration": float(np.median(loop_ms)) * 1e3 / max(1, int(np.median(iters))),
