@@ -1169,6 +1169,11 @@ struct Engine {
       x.dense = dense;
       x.ndual = ndual;
       x.nvstep = nvstep;
+      if (mri) {
+        const SliceDev& d0 = hs[groups[r.g][0]];
+        x.adj_fused = dense && x.adj_threads == kRowAdjThreadsWide && d0.ngr_a == 1 && d0.S != nullptr &&
+                      rows_adj_fused_ok(dt, d0.W, d0.C, d0.ncols) && (!tgv || nvstep > 0);
+      }
       if (!(dense && kind == PDT_SOLVER_TV)) x.tb_x = x.tb_y = x.tb_k = 0;
       x.tb_w = hs[groups[r.g][0]].W;
       x.tb_h = hs[groups[r.g][0]].H;

@@ -53,6 +53,10 @@ template <class R> __device__ __forceinline__ Cx<R> operator*(Cx<R> a, Cx<R> b) 
 template <class R> __device__ __forceinline__ Cx<R> operator*(R s, Cx<R> a) { return {s * a.re, s * a.im}; }
 template <class R> __device__ __forceinline__ Cx<R> operator*(Cx<R> a, R s) { return {s * a.re, s * a.im}; }
 template <class R> __device__ __forceinline__ Cx<R> operator/(Cx<R> a, R s) { return {a.re / s, a.im / s}; }
+// c ? a : b, component-wise (a select, not a branch)
+template <class R> __device__ __forceinline__ Cx<R> csel(bool c, Cx<R> a, Cx<R> b) {
+  return {c ? a.re : b.re, c ? a.im : b.im};
+}
 // a * conj(b)
 template <class R> __device__ __forceinline__ Cx<R> mulconj(Cx<R> a, Cx<R> b) {
   return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
@@ -1305,13 +1309,12 @@ template <class T, int NF, int NT>
 struct Staged {
   static constexpr int K = NF > 0 ? (NF + NT - 1) / NT : 1;
   T v[K];
+  // branch-free (lanes past n load element n - 1, never stored), so that the
+  // compiler's waits count these loads exactly
   template <class A>
   __device__ __forceinline__ void load(A* src, int n) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int e = threadIdx.x + k * NT;
-      if (e < n) v[k] = src[e];
-    }
+    for (int k = 0; k < K; ++k) v[k] = src[min(int(threadIdx.x) + k * NT, n - 1)];
   }
   __device__ __forceinline__ void store(T* dst, int n) const {
 #pragma unroll
@@ -1454,11 +1457,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GAP ? 5 : 1)
 
 // Column pass.  CMODE: 0 solver (forward, r prox, adjoint), 1 operator
 // forward (writes op_out {W, Hk, C}), 2 operator adjoint (reads op_in).
+// PF (solver, compile-time length, at most kColPrefetchCols columns per
+// item): the gather and the prox operands go through registers.  It is a
+// template argument so that the other gather, a loop, is not compiled into
+// the same kernel: the compiler's waits after the join of the two would count
+// the prox loads as outstanding before the FFT.
 enum { CM_SOLVE = 0, CM_FWD = 1, CM_ADJ = 2 };
 
-template <class R, int CMODE, bool GAP, int NT, int NF>
+template <class R, int CMODE, bool GAP, int NT, int NF, bool PF = false>
 __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* items, int nitems, int cg,
                                              double* part) {
+  static_assert(!PF || (NF > 0 && CMODE == CM_SOLVE), "register prefetch needs a compile-time length");
   extern __shared__ char smem_raw[];
   const int li = xcd_item(blockIdx.x, nitems);
   const int2 it = items ? items[li] : make_int2(0, li);  // null: the dense single-slice list
@@ -1481,8 +1490,9 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   double t[2] = {0, 0};
   // TGV_prox_r_kernel (tgv_kernels.cuh:212-223) operands; columns ks0.. of
   // coil c are contiguous in the compact layout.  With a compile-time length
-  // they are loaded into registers here, together with the T1 gather, so that
-  // their latency is not paid again after the forward FFT.
+  // they are loaded into registers, issued after the T1 gather: vector memory
+  // loads return in issue order, so the gather's wait before the LDS stores
+  // leaves them in flight, and they arrive during the forward FFT.
   const R sigma = R(d.sigma), lambda = R(d.lambda), inv_sl = rcp(R(1) + sigma * lambda);
   const auto fk = gptr<const C>(d.fk) + (size_t(c) * nc + ks0) * Hk;
   const auto rk = gptr<C>(d.rk) + (size_t(c) * nc + ks0) * Hk;
@@ -1490,34 +1500,36 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   constexpr int KP = NF > 0 ? (kColPrefetchCols * NF + NT - 1) / NT : 1;
   C pf_f[KP], pf_r[KP];
   R pf_m[KP];
-  const bool pf = NF > 0 && CMODE == CM_SOLVE && ncg <= kColPrefetchCols;
+  constexpr bool pf = PF;
+  // the gather of the zero-padded, ifftshifted columns of T1 (coil c, columns
+  // ks0.., contiguous in y): into registers with a compile-time length
+  const auto T1 = gptr<const C>(d.T1) + (size_t(c) * nc + ks0) * H;
+  C pf_t[KP];
   if (pf) {
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
       const int e = threadIdx.x + k * NT;
       if (e < ncg * Hk) {
-        pf_f[k] = fk[e];
-        pf_r[k] = rk[e];
-        pf_m[k] = mk[e];
+        const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
+        const int yy = ishift_src(j, Hk) - y0;
+        pf_t[k] = (yy >= 0 && yy < H) ? C(T1[cl * H + yy]) : pzero<C>();
       }
     }
-  }
-  if (!slice_active(d)) return;  // after the prefetch is issued, before any store
-  if (CMODE != CM_ADJ) {
-    const auto T1 = gptr<const C>(d.T1) + (size_t(c) * nc + ks0) * H;  // coil c, column ks0 (contiguous in y)
-    // the gather of the zero-padded, ifftshifted columns: into registers with
-    // the prefetch (one latency for both), else straight into LDS
-    C pf_t[KP];
-    if (pf) {
+    // branch-free (lanes past the columns load the last element, unused), so
+    // that the compiler's wait for the gather counts these loads as issued
+    const int last = ncg * Hk - 1;
 #pragma unroll
-      for (int k = 0; k < KP; ++k) {
-        const int e = threadIdx.x + k * NT;
-        if (e < ncg * Hk) {
-          const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
-          const int yy = ishift_src(j, Hk) - y0;
-          pf_t[k] = (yy >= 0 && yy < H) ? C(T1[cl * H + yy]) : pzero<C>();
-        }
-      }
+    for (int k = 0; k < KP; ++k) {
+      const int e = min(int(threadIdx.x) + k * NT, last);
+      pf_f[k] = fk[e];
+      pf_r[k] = rk[e];
+      pf_m[k] = mk[e];
+    }
+  }
+  if (!slice_active(d)) return;  // after the loads are issued, before any store
+  if (CMODE != CM_ADJ) {
+    // registers -> LDS with the prefetch, else the gather straight into LDS
+    if (pf) {
 #pragma unroll
       for (int k = 0; k < KP; ++k) {
         const int e = threadIdx.x + k * NT;
@@ -1616,6 +1628,118 @@ __device__ __forceinline__ void mri_primal_pixel(const SliceDev& d, int x, int y
   }
 }
 
+// Register slots per thread of the adjoint row pass: the 1024-thread variant
+// must stay within 64 VGPRs (two blocks per CU), so it holds fewer.
+template <int NT> constexpr int rows_adj_ps() { return NT >= 1024 ? 6 : kPrefetch; }
+template <int NT> constexpr int rows_adj_pt() { return NT >= 1024 ? 2 : 4; }
+// the straight-line path loads its sensitivity slots unconditionally, so it
+// holds exactly as many as it may use (5 coils per thread group: 15 coils
+// on rows of 320, 20 on rows of 256)
+constexpr int kRowsAdjFusedSlots = 5;
+
+// The adjoint row pass of one row y holding all coils (FUSED in k_rows_adj):
+// T2 row -> ifftshift -> inverse PE-FFT -> fftshift -> * conj(S_c), summed
+// over the coils (thread group gi takes coils [gi ch, (gi + 1) ch) of pixel
+// gx, the group sums added in group order), then the u step of row y
+// (TGV_primal_u_noprox_kernel, tgv_kernels.cuh:432-467).  Load order: the
+// twiddles, the T2 gather, u and p of the primal step (their arithmetic
+// waits for them), then the sensitivities, which the FFT does not need.
+template <class R, int MODE, bool GAP, int NT, int NF>
+__device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double* part_rows, char* smem_raw) {
+  using C = Cx<R>;
+  constexpr int W = NF, PS = kRowsAdjFusedSlots, PT = rows_adj_pt<NT>();
+  const int H = d.H, nc = d.ncols, ncb = d.C;
+  RowLds<C> L(smem_raw, W, ncb, false);
+  Staged<C, NF, NT> s_tw;
+  s_tw.load(gptr<const C>(d.planW.tw), W);
+  for (int e = threadIdx.x; e < ncb * W; e += NT) L.a[e] = pzero<C>();
+  // T2 row y of every coil and the PE column of each sampled entry
+  const float rnc = 1.0f / float(max(nc, 1));
+  const auto T2 = gptr<const C>(d.T2) + size_t(y) * nc;
+  const auto gcols = gptr<const int>(d.cols);
+  const int cstride = H * nc;
+  C pT[PT];
+  int pK[PT];
+#pragma unroll
+  for (int k = 0; k < PT; ++k) {
+    const int e = threadIdx.x + k * NT;
+    if (e < ncb * nc) {
+      const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
+      pT[k] = T2[cl * cstride + ks];
+      pK[k] = gcols[ks];
+    }
+  }
+  // u(x, y) and the neighbours of div p (p is final for this iteration),
+  // loaded branch-free (threads past the row load pixel 0, unused)
+  const R tau = R(d.tau), theta = R(d.theta);
+  const int x = threadIdx.x, xr = x < W ? x : 0;
+  const size_t n = size_t(W) * H, i = size_t(y) * W + xr;
+  const auto pp = gptr<const C>(d.p);
+  const C e_uo = gptr<const C>(d.u)[i];
+  const C cx = pp[i], mx = pp[xr > 0 ? i - 1 : i];
+  const C cy = pp[n + i], my = pp[n + (y > 0 ? i - W : i)];
+  // sensitivities, branch-free (spare slots load a valid element, unused)
+  const int ngs = min(NT / W, ncb), ch = (ncb + ngs - 1) / ngs;
+  const int gi = threadIdx.x / W, gx = threadIdx.x - gi * W;
+  const int G = W * d.Hk;
+  const auto Sg = gptr<const C>(d.S) + size_t(y + d.y0) * W + gx;
+  C pS[PS];
+#pragma unroll
+  for (int k = 0; k < PS; ++k) pS[k] = Sg[size_t(min(gi * ch + k, ncb - 1)) * G];
+  if (!slice_active(d)) return;  // after the loads are issued, before any store
+  // descriptor fields of the stores, read before the asm below (after it
+  // the compiler no longer takes the descriptor for unmodified, and would
+  // read them with vector loads at the end of the block)
+  const auto gu = gptr<C>(d.u);
+  const auto gub = gptr<C>(d.ub);
+  const auto gctl = gptr<int>(d.ctl);
+  const int row_beg = d.row_beg;
+  // div p = dmx + dmy (the same operations as dmx / dmy), formed here: the
+  // empty asm keeps the compiler from sinking it past the FFT, where the
+  // four neighbours would hold registers instead of the one sum
+  const C dx = csel(xr > 0, csel(xr < W - 1, cx - mx, -mx), cx);
+  const C dy = csel(y > 0, csel(y < H - 1, cy - my, -my), cy);
+  C e_div = dx + dy;
+  asm volatile("" : "+v"(e_div.re), "+v"(e_div.im));
+  s_tw.store(L.tw, W);
+  __syncthreads();  // zero fill, twiddles
+#pragma unroll
+  for (int k = 0; k < PT; ++k) {
+    const int e = threadIdx.x + k * NT;
+    if (e < ncb * nc) {
+      const int cl = qdiv(e, nc, rnc);
+      L.a[lp<NF, false>(cl * W + ishift_dst(pK[k], W))] = pT[k];
+    }
+  }
+  __syncthreads();
+  C* res = fft_n<R, NT, true, NF, false>(L.a, L.b, ncb, d.planW, L.tw);
+  C* other = (res == L.a) ? L.b : L.a;
+  if (gi < ngs) {
+    C acc = pzero<C>();
+#pragma unroll
+    for (int k = 0; k < PS; ++k) {
+      const int cl = gi * ch + k;
+      if (k < ch && cl < ncb) acc = acc + mulconj(res[lp<NF, false>(cl * W + fshift_src(gx, W))], pS[k]);
+    }
+    other[gi * W + gx] = acc;
+  }
+  __syncthreads();
+  double t[2] = {0, 0};
+  if (x < W) {
+    C w = other[x];
+    for (int q = 1; q < ngs; ++q) w = w + other[q * W + x];
+    const C un = e_uo + tau * (e_div - w);
+    gu[i] = un;
+    gub[i] = (R(1) + theta) * un - theta * e_uo;
+    if (GAP) t[0] += dabs2(w - e_div);
+  }
+  if (GAP) block_sum<2, NT>(t, part_rows + size_t(row_beg + y) * kGapTerms);
+  if (y == 0 && threadIdx.x == 0) {
+    gctl[CTL_ITER] += 1;
+    gctl[CTL_RUPD] += 1;
+  }
+}
+
 // Row pass, adjoint.  Item = (slice, y * ngr_a + g): for each coil of group g,
 // T2 row -> ifftshift -> inverse PE-FFT -> fftshift -> * conj(S_c), summed
 // over the group's coils in order.  With several coil groups (ngr_a > 1) the
@@ -1624,11 +1748,20 @@ __device__ __forceinline__ void mri_primal_pixel(const SliceDev& d, int x, int y
 // row y itself (MODE M_OP: writes A^* r to op_out).  The 1024-thread
 // variant on swz-permuted buffers is held to 64 VGPRs (its slot arithmetic
 // would take more), so that two workgroups share a CU.
-template <class R, int MODE, bool GAP, int NT, int NF>
+// FUSED (host-checked, see rows_adj_fused_ok): a compile-time row length, one
+// coil group per row holding every coil (the u step follows in the block),
+// 2 W <= NT, the coil products in registers (every thread group's coils fit
+// its slots), the T2 row in registers, sensitivities present, and no v step
+// in the row blocks (TGV: separate workgroups).  That path is straight-line,
+// so the compiler's waits can leave the sensitivity loads, issued after the
+// T2 gather and the primal-step loads, in flight during the inverse FFT (the
+// other paths' runtime branches make it wait for every load before it).
+template <class R, int MODE, bool GAP, int NT, int NF, bool FUSED = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1024 && NF > 0 && ct_pad_ok(NF) ? 8 : 1))) void k_rows_adj(const SliceDev* sl, const int2* items, int nitems, int cb,
                                                  double* part_rows, int nvstep) {
   extern __shared__ char smem_raw[];
   using C = Cx<R>;
+  static_assert(!FUSED || (NF > 0 && MODE != M_OP && NT == kRowAdjThreadsWide), "fused adjoint row path");
   if (int(blockIdx.x) >= nitems) {
     // nvstep workgroups after the row items (a single slice): the v step of
     // TGV_primal_v_kernel on contiguous pixel ranges, where the row items
@@ -1656,6 +1789,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1024 &
   const int li = xcd_item(blockIdx.x, nitems);
   const int2 it = items ? items[li] : make_int2(0, li);  // null: the dense single-slice list
   const SliceDev& d = sl[it.x];
+  if constexpr (FUSED) {
+    rows_adj_fused<R, MODE, GAP, NT, NF>(d, it.y, part_rows, smem_raw);
+    return;
+  }
   const int W = NF > 0 ? NF : d.W, H = d.H, nc = d.ncols;
   const int y = it.y / d.ngr_a, g = it.y - y * d.ngr_a;
   const int c0 = g * cb, ncb = min(cb, d.C - c0);
@@ -1674,9 +1811,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1024 &
   const auto S = gptr<const C>(d.S);
   const int G = W * d.Hk;
   const auto Sg = S ? S + size_t(c0) * G + size_t(y + d.y0) * W : S;
-  // register slots per thread: the 1024-thread variant must stay within 64
-  // VGPRs (two blocks per CU), so it holds fewer
-  constexpr int PS = NT >= 1024 ? 6 : kPrefetch, PT = NT >= 1024 ? 2 : 4;
+  constexpr int PS = rows_adj_ps<NT>(), PT = rows_adj_pt<NT>();
   C pS[PS];
   // Coil sum in registers (one block holds the whole row, u step fused):
   // thread t = gi W + x takes pixel x of the coils [gi ch, (gi + 1) ch) of
@@ -2191,6 +2326,13 @@ bool fft_len_compiled(int dtype, int n) {
   return false;
 }
 
+bool rows_adj_fused_ok(int dtype, int W, int C, int nc) {
+  constexpr int NW = kRowAdjThreadsWide;
+  if (!fft_len_compiled(dtype, W) || 2 * W > NW || C < 1) return false;
+  const int ngs = std::min(NW / W, C), ch = (C + ngs - 1) / ngs;
+  return ch <= kRowsAdjFusedSlots && C * nc <= rows_adj_pt<NW>() * NW;
+}
+
 // Adjoint row pass with L.adj_threads threads per block (kRowThreads, or
 // kRowAdjThreadsWide for items holding many coils).
 template <class R, int MODE, int NF>
@@ -2199,6 +2341,13 @@ static void launch_rows_adj(const Launch& L, bool gap) {
   const size_t lds = lds_occupancy(L.lds_rows_a, L.wpc[WPC_ADJ], init_kernel_attrs());
   if (L.adj_threads == NW) {
     const int nv = L.dense ? L.nvstep : 0;
+    if constexpr (NF > 0 && MODE != M_OP) {
+      if (L.adj_fused) {
+        if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NW, NF, true>), L.nrg_a + nv, NW, lds, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, nv);
+        else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NW, NF, true>), L.nrg_a + nv, NW, lds, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, nv);
+        return;
+      }
+    }
     if (gap) PDT_LAUNCH_NT((k_rows_adj<R, MODE, true, NW, NF>), L.nrg_a + nv, NW, lds, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, nv);
     else PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NW, NF>), L.nrg_a + nv, NW, lds, L.stream, L.sl, L.dense ? nullptr : L.rowgroups_a, L.nrg_a, L.cb_a, L.gap_rows, nv);
   } else {
@@ -2225,8 +2374,16 @@ static void mri_stage_m(const Launch& L, bool gap, int stage) {
     const size_t lds = lds_occupancy(L.lds_cols, L.wpc[WPC_COLS], init_kernel_attrs());
     with_fft_len<R>(L.fft_h, [&](auto nf) {
       constexpr int NF = decltype(nf)::value;
-      if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, L.dense ? nullptr : L.cols, L.ncolitems, L.cg, L.part[1]);
-      else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, L.dense ? nullptr : L.cols, L.ncolitems, L.cg, L.part[1]);
+      const int2* items = L.dense ? nullptr : L.cols;
+      if constexpr (NF > 0) {
+        if (L.cg <= kColPrefetchCols) {
+          if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF, true>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1]);
+          else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF, true>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1]);
+          return;
+        }
+      }
+      if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1]);
+      else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1]);
     });
   } else if (stage == 2) {
     with_fft_len<R>(L.fft_w, [&](auto nf) {
@@ -2332,8 +2489,18 @@ static bool set_lds_attrs(int b) {
     set((const void*)&k_rows_adj<R, M_TV, true, kRowAdjThreadsWide, NF>);
     set((const void*)&k_rows_adj<R, M_TGV, false, kRowAdjThreadsWide, NF>);
     set((const void*)&k_rows_adj<R, M_TGV, true, kRowAdjThreadsWide, NF>);
+    if constexpr (NF > 0) {
+      set((const void*)&k_rows_adj<R, M_TV, false, kRowAdjThreadsWide, NF, true>);
+      set((const void*)&k_rows_adj<R, M_TV, true, kRowAdjThreadsWide, NF, true>);
+      set((const void*)&k_rows_adj<R, M_TGV, false, kRowAdjThreadsWide, NF, true>);
+      set((const void*)&k_rows_adj<R, M_TGV, true, kRowAdjThreadsWide, NF, true>);
+    }
     set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF>);
     set((const void*)&k_cols<R, CM_SOLVE, true, NC, NF>);
+    if constexpr (NF > 0) {
+      set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF, true>);
+      set((const void*)&k_cols<R, CM_SOLVE, true, NC, NF, true>);
+    }
     set((const void*)&k_cols<R, CM_FWD, false, NC, NF>);
     set((const void*)&k_cols<R, CM_ADJ, false, NC, NF>);
   };

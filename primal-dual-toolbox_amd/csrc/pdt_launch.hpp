@@ -48,6 +48,10 @@ struct Launch {
   // single dense slice, TGV, wide adjoint blocks: workgroups of k_rows_adj
   // that do the v step alone after the row items (0: the row items do it)
   int nvstep = 0;
+  // single dense slice, wide adjoint blocks, one coil group per row: the
+  // straight-line adjoint row path (rows_adj_fused_ok; no v step in the row
+  // blocks)
+  bool adj_fused = false;
   // workgroup size of the denoising half-iterations (256, 512 or 1024): the
   // smallest that covers rpi rows of the widest slice in one pass, at most 1024
   int den_threads = 256;
@@ -119,6 +123,9 @@ void run_sumsq(hipStream_t st, int dtype, const void* z, int64_t n, double* out)
 // True when the row/column kernels of this precision have a compile-time
 // FFT plan for length n (the kernels fall back to the runtime plan otherwise).
 bool fft_len_compiled(int dtype, int n);
+// Whether the straight-line adjoint row path fits a slice with rows of W
+// pixels, C coils (all in one coil group) and nc listed PE columns.
+bool rows_adj_fused_ok(int dtype, int W, int C, int nc);
 
 // Allows the MRI kernels the full LDS of a workgroup (call before launches);
 // returns the dynamic LDS bytes a workgroup may use.
