@@ -293,7 +293,7 @@ def extra_tv256(steps: int, cpu_threads: int, no_cpu: bool, autotune: bool = Tru
     return out
 
 
-def extra_tgv512(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
+def extra_tgv512(steps: int, cpu_threads: int, no_cpu: bool, autotune: bool = True) -> dict:
     """TGV^2 denoising, 512^2 complex64, lambda 20, alpha0 2, alpha1 1, 1000
     iterations, FP32 and FP16 dual storage; accuracy against the FP64 GPU
     solver on the same input."""
@@ -302,12 +302,16 @@ def extra_tgv512(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
     from primaldualtoolbox import synthetic as syn
     _, noisy = syn.tgv_denoising_case(seed=0, n=512, sigma=0.05)
     res = {}
+    tiles = {}
     for name, dt, fp16 in (("f32", np.complex64, 0), ("f16dual", np.complex64, 1), ("f64", np.complex128, 0)):
         opt = pdt.denoising.TgvOptimizer_2c(dtype=dt)
         opt.setNoisyData(noisy.astype(dt))
         p = opt.getParameters()
         p.Lambda, p.alpha0, p.alpha1, p.max_iter, p.dual_fp16, p.gap_every = 20, 2, 1, 999, fp16, 1000
         f_dev = C.DeviceArray.from_numpy(noisy.astype(dt), dt)
+        if autotune:
+            tiles[name] = pdt.denoising.autotune([opt], reps=20)
+            opt.setNoisyData(noisy.astype(dt))
         opt.solve()
         res[name] = (opt, _timed_solves(opt, f_dev, steps), opt.getResult().copy())
     ref = res["f64"][2]
@@ -315,7 +319,7 @@ def extra_tgv512(steps: int, cpu_threads: int, no_cpu: bool) -> dict:
     for name in ("f32", "f16dual"):
         opt, s, u = res[name]
         out[name] = {"seconds": s, "us_per_iteration": s * 1e6 / 1000, "rel_l2_vs_f64": _rel(u, ref),
-                     "rel_gap": float(opt.getStats()["rel_gap"]),
+                     "rel_gap": float(opt.getStats()["rel_gap"]), "tiles": tiles.get(name),
                      "kernels": _kernels(opt.profileKernels(50))}
     out["seconds_f64"] = res["f64"][1]
     if not no_cpu:
@@ -504,7 +508,7 @@ def side_measurements(args, keys=None) -> dict:
     tgv_512, demo_384, batch (the batch of args.batch slices)."""
     runs = {"cnn": extra_cnn,
             "tv_256": lambda: extra_tv256(3, args.cpu_threads, args.no_cpu, not args.no_autotune),
-            "tgv_512": lambda: extra_tgv512(3, args.cpu_threads, args.no_cpu),
+            "tgv_512": lambda: extra_tgv512(3, args.cpu_threads, args.no_cpu, not args.no_autotune),
             "demo_384": lambda: extra_demo384(3, args.cpu_threads, args.no_cpu, not args.no_autotune,
                                               not args.no_graph),
             "batch": lambda: extra_batch64(args.batch, args.gen_workers, args.cpu_threads, args.no_cpu,

@@ -204,6 +204,7 @@ int max_cg(int dt, int Hk, int nc) {
 // tiles.
 constexpr int kDefaultCb = 4, kDefaultCg = 2;
 constexpr int kDefaultTvBlock[3] = {32, 16, 8};
+constexpr int kDefaultTgvBlock[3] = {16, 16, 4};
 
 }  // namespace
 
@@ -619,7 +620,9 @@ struct pdt_solver {
   bool have_u = false;
   std::unique_ptr<pdt_array> u, v;  // result handles (u: {W,H}, v: {W,H,2})
   DevBuf ub, vb, p, q, tmp;
-  DevBuf u2, ub2, p2;  // ROF: second copy of u, ubar, p for the temporally blocked launches
+  // second copy of the state for the temporally blocked denoising launches:
+  // u, ubar, p (ROF and TGV) and v, vbar, q (TGV)
+  DevBuf u2, ub2, p2, v2, vb2, q2;
   DevBuf fk, rk, T1, T2, wpart;
   // set-up buffers kept across solves: descriptor and item list of the data
   // compaction, its partial sums, A^*f, and two reduction results
@@ -690,10 +693,15 @@ void alloc_state(pdt_solver* s, int W, int H) {
     exact(s->q, 3 * n * s->pq_e());
     if (s->kind == PDT_SOLVER_TGV) exact(s->tmp, 2 * n * e);
   }
-  if (s->kind == PDT_SOLVER_TV) {
+  if (s->kind == PDT_SOLVER_TV || s->kind == PDT_SOLVER_TGV) {
     exact(s->u2, n * e);
     exact(s->ub2, n * e);
     exact(s->p2, 2 * n * s->pq_e());
+  }
+  if (s->kind == PDT_SOLVER_TGV) {
+    exact(s->v2, 2 * n * e);
+    exact(s->vb2, 2 * n * e);
+    exact(s->q2, 3 * n * s->pq_e());
   }
   s->W = W;
   s->H = H;
@@ -883,10 +891,13 @@ struct Engine {
       // double: 12.2 against 9.5 us per dual step)
       const int dpx = force_dpx > 0 ? force_dpx : tv.dual_px > 0 ? tv.dual_px : is_dbl(dtype) ? 1 : 2;
       pair_duals = dpx == 2;
-      // temporally blocked ROF launches: one slice only, tiles that fit the LDS
-      const int* tb = force_tb[2] >= 0 ? force_tb : tv.tb[2] >= 0 ? tv.tb : kDefaultTvBlock;
-      if (kind == PDT_SOLVER_TV && ns == 1 && tb[2] > 0 && tb[0] > 0 && tb[1] > 0 &&
-          tv_block_lds(dt, tb[0], tb[1], tb[2]) <= init_kernel_attrs()) {
+      // temporally blocked launches: one slice only, tiles that fit the LDS
+      const int* tb = force_tb[2] >= 0 ? force_tb
+                      : tv.tb[2] >= 0  ? tv.tb
+                      : kind == PDT_SOLVER_TGV ? kDefaultTgvBlock
+                                               : kDefaultTvBlock;
+      if ((kind == PDT_SOLVER_TV || kind == PDT_SOLVER_TGV) && ns == 1 && tb[2] > 0 && tb[0] > 0 && tb[1] > 0 &&
+          block_lds(kind, dt, tb[0], tb[1], tb[2]) <= init_kernel_attrs()) {
         L.tb_x = tb[0];
         L.tb_y = tb[1];
         L.tb_k = tb[2];
@@ -947,6 +958,9 @@ struct Engine {
       d.u2 = s->u2.p;
       d.ub2 = s->ub2.p;
       d.p2 = s->p2.p;
+      d.v2 = s->v2.p;
+      d.vb2 = s->vb2.p;
+      d.q2 = s->q2.p;
       d.f = s->f->ptr;
       if (mri) {
         d.fk = s->fk.p;
@@ -1174,7 +1188,7 @@ struct Engine {
         x.adj_fused = dense && x.adj_threads == kRowAdjThreadsWide && d0.ngr_a == 1 && d0.S != nullptr &&
                       rows_adj_fused_ok(dt, d0.W, d0.C, d0.ncols) && (!tgv || nvstep > 0);
       }
-      if (!(dense && kind == PDT_SOLVER_TV)) x.tb_x = x.tb_y = x.tb_k = 0;
+      if (!(dense && (kind == PDT_SOLVER_TV || kind == PDT_SOLVER_TGV))) x.tb_x = x.tb_y = x.tb_k = 0;
       x.tb_w = hs[groups[r.g][0]].W;
       x.tb_h = hs[groups[r.g][0]].H;
       listed_items += r.nrow + r.nrg + r.nrga + r.ncol;
@@ -1272,8 +1286,8 @@ struct Engine {
       const int k = G[0].tb_k;
       int plain = len - (gap ? 1 : 0);
       for (; plain >= 2 * k; plain -= 2 * k) {
-        run_tv_block(G[0], dt, 0);
-        run_tv_block(G[0], dt, 1);
+        run_block(G[0], kind, dt, fp16, 0);
+        run_block(G[0], kind, dt, fp16, 1);
       }
       for (int i = 0; i < plain; ++i) iteration(false);
       if (gap) {
@@ -1954,17 +1968,19 @@ void profile_engine(std::vector<pdt_solver*> sv, int reps, pdt_kernel_prof* out,
     e.build();
     e.reset_ctl();
     if (e.blocked()) {
-      // temporally blocked ROF launches, as a solve runs them: a graph of
-      // reps pairs of launches timed with an event pair; per launch (tb_k
-      // iterations): read u, ubar, p, f, write u, ubar, p once each
+      // temporally blocked launches, as a solve runs them: a graph of reps
+      // pairs of launches timed with an event pair; per launch (tb_k
+      // iterations) each plane read or written once: ROF reads u, ubar, p,
+      // f and writes u, ubar, p; TGV reads u, v, p, q, f and writes u, ubar,
+      // v, vbar, p, q
       hipEvent_t e0, e1;
       HIPC(hipEventCreate(&e0));
       HIPC(hipEventCreate(&e1));
       hipGraph_t graph;
       HIPC(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
       for (int r = 0; r < reps; ++r) {
-        run_tv_block(e.G[0], e.dtk(), 0);
-        run_tv_block(e.G[0], e.dtk(), 1);
+        run_block(e.G[0], e.kind, e.dtk(), e.fp16, 0);
+        run_block(e.G[0], e.kind, e.dtk(), e.fp16, 1);
       }
       HIPC(hipStreamEndCapture(e.stream, &graph));
       hipGraphExec_t ge;
@@ -1981,10 +1997,13 @@ void profile_engine(std::vector<pdt_solver*> sv, int reps, pdt_kernel_prof* out,
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       double bytes = 0;
-      for (auto* s : sv) bytes += 9.0 * double(s->W) * s->H * double(s->pe());
+      for (auto* s : sv) {
+        const double N = double(s->W) * s->H, pe = double(s->pe()), pq = double(s->pq_e());
+        bytes += e.kind == PDT_SOLVER_TGV ? (10 * pe + 10 * pq) * N : (5 * pe + 4 * pq) * N;
+      }
       *n = 0;
       if (cap > 0) {
-        std::snprintf(out[0].name, sizeof(out[0].name), "k_tv_block");
+        std::snprintf(out[0].name, sizeof(out[0].name), e.kind == PDT_SOLVER_TGV ? "k_tgv_block" : "k_tv_block");
         out[0].ms = double(loop_ms) / (2.0 * reps);
         out[0].bytes = bytes;
         *n = 1;
@@ -2097,12 +2116,12 @@ double time_iteration(Engine& e, int reps) {
     hipEvent_t e0, e1;
     HIPC(hipEventCreate(&e0));
     HIPC(hipEventCreate(&e1));
-    run_tv_block(e.G[0], e.dtk(), 0);
-    run_tv_block(e.G[0], e.dtk(), 1);
+    run_block(e.G[0], e.kind, e.dtk(), e.fp16, 0);
+    run_block(e.G[0], e.kind, e.dtk(), e.fp16, 1);
     HIPC(hipEventRecord(e0, e.stream));
     for (int r = 0; r < reps; ++r) {
-      run_tv_block(e.G[0], e.dtk(), 0);
-      run_tv_block(e.G[0], e.dtk(), 1);
+      run_block(e.G[0], e.kind, e.dtk(), e.fp16, 0);
+      run_block(e.G[0], e.kind, e.dtk(), e.fp16, 1);
     }
     HIPC(hipEventRecord(e1, e.stream));
     HIPC(hipEventSynchronize(e1));
@@ -2129,9 +2148,9 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
     // Phase 1, tile shapes without occupancy caps: coils per forward row
     // item, columns per column item, rows per denoising item, coils per
     // adjoint row item (the forward choice, or every coil of the row, which
-    // also fuses the u step into that pass); for a single ROF slice also
-    // temporally blocked launches (interior tile, iterations per launch),
-    // compared per iteration
+    // also fuses the u step into that pass); for a single ROF or TGV
+    // denoising slice also temporally blocked launches (interior tile,
+    // iterations per launch), compared per iteration
     struct Cand {
       std::array<int, 4> t;
       std::array<int, 3> tb;
@@ -2158,6 +2177,12 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
         for (const auto& tb : std::vector<std::array<int, 3>>{
                  {16, 16, 4}, {16, 16, 6}, {32, 16, 4}, {32, 16, 8}, {32, 32, 8}, {64, 32, 8}, {32, 32, 12},
                  {64, 64, 8}, {64, 64, 16}})
+          cand.push_back({{0, 0, 1, 0}, tb, 0});
+      // TGV holds 12 planes per tile pixel: smaller tiles and halos
+      if (sv[0]->kind == PDT_SOLVER_TGV && sv.size() == 1)
+        for (const auto& tb : std::vector<std::array<int, 3>>{
+                 {16, 8, 2}, {16, 16, 2}, {16, 16, 4}, {32, 8, 2}, {32, 16, 2}, {32, 16, 4}, {32, 32, 2}, {32, 32, 4},
+                 {64, 16, 4}, {64, 32, 3}})
           cand.push_back({{0, 0, 1, 0}, tb, 0});
     }
     auto engine = [&](const Cand& c, const int (&wpc)[4]) {

@@ -78,11 +78,15 @@ struct SliceDev {
   void* q;   // 3 planes (xx, yy, xy)
   const void* f;  // denoising data
   void* tmp;      // 2 planes: TGV dual-restoration scratch
-  // the second copy of u, ubar, p (2 planes) that the temporally blocked ROF
-  // launches write and read in turn (ping-pong)
+  // the second copy of the state that the temporally blocked denoising
+  // launches write and read in turn (ping-pong): u, ubar, p (2 planes) and,
+  // for TGV, v, vbar (2 planes each) and q (3 planes)
   void* u2;
   void* ub2;
   void* p2;
+  void* v2;
+  void* vb2;
+  void* q2;
   // geometry
   int W, H;       // image {W (x, contiguous), H (y)}
   int Hk, y0, C;  // k-space FE length, FE offset of the image window, coils

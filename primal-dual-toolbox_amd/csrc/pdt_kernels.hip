@@ -1079,6 +1079,175 @@ __global__ __launch_bounds__(NT) void k_tv_block(const SliceDev* sl, int tx, int
   if (blockIdx.x == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += k;
 }
 
+// The value a dual field holds after a store to its storage type S and a
+// load back (S = P: itself; fp16 storage: rounded to half).
+template <class P, class S>
+__device__ __forceinline__ P storage_round(P v) {
+  S s[1];
+  Ld<P, S>::put(s, 0, v);
+  return Ld<P, S>::get(s, 0);
+}
+
+// Temporally blocked TGV denoising: tb_k times the primal step
+// (TV_primal_u_kernel + TGV_primal_v_kernel) then the dual step
+// (TGV_dual_p_kernel + TGV_dual_q_kernel) per launch, in the reference's
+// order (tgvoptimizer.cu:119-135; tgv_kernels.cuh:225-292,568-605,647-686),
+// on a tile held in LDS.  Block (bx, by) owns the interior [x0, x1) x
+// [y0, y1) and loads u, v (2 planes), p (2), q (3) and f on the interior
+// grown by k pixels on every side (clipped to the image).  The primal step at
+// (x, y) needs p, q at x - 1 and y - 1 (backward differences, none at the
+// first column / row), the dual step ubar, vbar at x + 1 and y + 1 (forward
+// differences, none at the last): the region where the values are exact
+// shrinks by one pixel per step on each side that is not an image border,
+// and after k iterations the interior is exact.  It is written with ubar and
+// vbar to the other copy of the state (src -> dst).  The per-pixel arithmetic
+// is that of k_tgv_primal / k_tgv_dual in the same order, and the duals are
+// rounded to their storage type after every dual step (as a store and a load
+// would round them), so the iterates equal theirs bit for bit.
+template <class P, class S, int NT>
+__global__ __launch_bounds__(NT) void k_tgv_block(const SliceDev* sl, int tx, int ty, int k, int dir) {
+  extern __shared__ char smem_raw[];
+  using R = typename RealOf<P>::T;
+  using L = Ld<P, S>;
+  const SliceDev& d = sl[0];
+  if (!slice_active(d)) return;
+  const int W = d.W, H = d.H;
+  const int nbx = (W + tx - 1) / tx;
+  const int b = xcd_item(blockIdx.x, gridDim.x);  // neighbouring tiles on one XCD
+  const int by = b / nbx, bx = b - by * nbx;
+  const int x0 = bx * tx, x1 = min(W, x0 + tx), y0 = by * ty, y1 = min(H, y0 + ty);
+  const int xa = max(0, x0 - k), xb = min(W, x1 + k), ya = max(0, y0 - k), yb = min(H, y1 + k);
+  const int LW = xb - xa, LN = LW * (yb - ya);
+  P* sU = reinterpret_cast<P*>(smem_raw);
+  P* sUB = sU + LN;
+  P* sVX = sUB + LN;
+  P* sVY = sVX + LN;
+  P* sVBX = sVY + LN;
+  P* sVBY = sVBX + LN;
+  P* sPX = sVBY + LN;
+  P* sPY = sPX + LN;
+  P* sQXX = sPY + LN;
+  P* sQYY = sQXX + LN;
+  P* sQXY = sQYY + LN;
+  P* sF = sQXY + LN;
+  const size_t n = size_t(W) * H;
+  {
+    const auto gu = gptr<const P>(dir ? d.u2 : d.u);
+    const auto gv = gptr<const P>(dir ? d.v2 : d.v);
+    const auto gp = gptr<const S>(dir ? d.p2 : d.p);
+    const auto gq = gptr<const S>(dir ? d.q2 : d.q);
+    const auto gf = gptr<const P>(d.f);
+    const float rLW = 1.0f / float(LW);
+    for (int e = threadIdx.x; e < LN; e += NT) {
+      const int ly = qdiv(e, LW, rLW), lx = e - ly * LW;
+      const size_t gi = size_t(ya + ly) * W + (xa + lx);
+      const P u = gu[gi], vx = gv[gi], vy = gv[gi + n], f = gf[gi];
+      const P px = L::get(gp, gi), py = L::get(gp, gi + n);
+      const P qxx = L::get(gq, gi), qyy = L::get(gq, gi + n), qxy = L::get(gq, gi + 2 * n);
+      sU[e] = u;
+      sVX[e] = vx;
+      sVY[e] = vy;
+      sF[e] = f;
+      sPX[e] = px;
+      sPY[e] = py;
+      sQXX[e] = qxx;
+      sQYY[e] = qyy;
+      sQXY[e] = qxy;
+    }
+  }
+  __syncthreads();
+  const R sigma = R(d.sigma), tau = R(d.tau), lambda = R(d.lambda), theta = R(d.theta);
+  const R ia0 = rcp(R(d.alpha0)), ia1 = rcp(R(d.alpha1));
+  // backward differences dmx / dmy on the tile
+  auto bdx = [&](const P* a, int l, int x) {
+    const P c = a[l], m = a[x > 0 ? l - 1 : l];
+    const P dd = (x < W - 1) ? c - m : -m;
+    return x > 0 ? dd : c;
+  };
+  auto bdy = [&](const P* a, int l, int y) {
+    const P c = a[l], m = a[y > 0 ? l - LW : l];
+    const P dd = (y < H - 1) ? c - m : -m;
+    return y > 0 ? dd : c;
+  };
+  int pxa = xa, pxb = xb, pya = ya, pyb = yb;  // where p, q are exact
+  for (int it = 0; it < k; ++it) {
+    // primal step where p, q and their backward neighbours are exact
+    const int uxa = pxa == 0 ? 0 : pxa + 1, uxb = pxb, uya = pya == 0 ? 0 : pya + 1, uyb = pyb;
+    {
+      const int rw = uxb - uxa, cnt = rw * (uyb - uya);
+      const float rr = 1.0f / float(max(rw, 1));
+      for (int e = threadIdx.x; e < cnt; e += NT) {
+        const int ry = qdiv(e, rw, rr), x = uxa + (e - ry * rw), y = uya + ry;
+        const int l = (y - ya) * LW + (x - xa);
+        const P uo = sU[l];
+        const P div = bdx(sPX, l, x) + bdy(sPY, l, y);
+        P un = uo + tau * (div + lambda * sF[l]);
+        un = un * rcp(R(1) + tau * lambda);
+        sU[l] = un;
+        sUB[l] = (R(1) + theta) * un - theta * uo;
+        const P dsx = bdx(sQXX, l, x) + bdy(sQXY, l, y);
+        const P dsy = bdy(sQYY, l, y) + bdx(sQXY, l, x);
+        const P ox = sVX[l], oy = sVY[l];
+        const P nx = ox + tau * (dsx + sPX[l]);
+        const P ny = oy + tau * (dsy + sPY[l]);
+        sVX[l] = nx;
+        sVY[l] = ny;
+        sVBX[l] = (R(1) + theta) * nx - theta * ox;
+        sVBY[l] = (R(1) + theta) * ny - theta * oy;
+      }
+    }
+    __syncthreads();
+    // dual step where ubar, vbar and their forward neighbours are exact
+    pxa = uxa;
+    pxb = uxb == W ? W : uxb - 1;
+    pya = uya;
+    pyb = uyb == H ? H : uyb - 1;
+    {
+      const int rw = pxb - pxa, cnt = rw * (pyb - pya);
+      const float rr = 1.0f / float(max(rw, 1));
+      for (int e = threadIdx.x; e < cnt; e += NT) {
+        const int ry = qdiv(e, rw, rr), x = pxa + (e - ry * rw), y = pya + ry;
+        const int l = (y - ya) * LW + (x - xa);
+        const int lx = l + (x < W - 1), ly = l + ((y < H - 1) ? LW : 0);
+        P px = sPX[l], py = sPY[l], qxx = sQXX[l], qyy = sQYY[l], qxy = sQXY[l];
+        dual_math<P, true, false>(sUB[l], sUB[lx], sUB[ly], sVBX[l], sVBY[l], sVBX[lx], sVBY[lx], sVBX[ly], sVBY[ly],
+                                  px, py, qxx, qyy, qxy, sigma, ia0, ia1, nullptr);
+        sPX[l] = storage_round<P, S>(px);
+        sPY[l] = storage_round<P, S>(py);
+        sQXX[l] = storage_round<P, S>(qxx);
+        sQYY[l] = storage_round<P, S>(qyy);
+        sQXY[l] = storage_round<P, S>(qxy);
+      }
+    }
+    __syncthreads();
+  }
+  const auto ou = gptr<P>(dir ? d.u : d.u2);
+  const auto oub = gptr<P>(dir ? d.ub : d.ub2);
+  const auto ov = gptr<P>(dir ? d.v : d.v2);
+  const auto ovb = gptr<P>(dir ? d.vb : d.vb2);
+  const auto op = gptr<S>(dir ? d.p : d.p2);
+  const auto oq = gptr<S>(dir ? d.q : d.q2);
+  const int iw = x1 - x0, icnt = iw * (y1 - y0);
+  const float riw = 1.0f / float(iw);
+  for (int e = threadIdx.x; e < icnt; e += NT) {
+    const int ry = qdiv(e, iw, riw), x = x0 + (e - ry * iw), y = y0 + ry;
+    const int l = (y - ya) * LW + (x - xa);
+    const size_t gi = size_t(y) * W + x;
+    ou[gi] = sU[l];
+    oub[gi] = sUB[l];
+    ov[gi] = sVX[l];
+    ov[gi + n] = sVY[l];
+    ovb[gi] = sVBX[l];
+    ovb[gi + n] = sVBY[l];
+    L::put(op, gi, sPX[l]);
+    L::put(op, gi + n, sPY[l]);
+    L::put(oq, gi, sQXX[l]);
+    L::put(oq, gi + n, sQYY[l]);
+    L::put(oq, gi + 2 * n, sQXY[l]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += k;
+}
+
 // TGV denoising primal half (tgvoptimizer.cu:119-127): TV_primal_u_kernel
 // then TGV_primal_v_kernel at every pixel.
 template <class P, class S, int NT>
@@ -2288,32 +2457,44 @@ static void with_fft_len(int n, Fn&& fn) {
   fn(std::integral_constant<int, 0>{});
 }
 
-size_t tv_block_lds(int dtype, int tx, int ty, int k) {
+size_t block_lds(int kind, int dtype, int tx, int ty, int k) {
   const size_t e = dtype == DT_F32 ? 4 : (dtype == DT_C128 ? 16 : 8);
-  return size_t(5) * size_t(tx + 2 * k) * size_t(ty + 2 * k) * e;
+  const size_t planes = kind == K_TGV ? 12 : 5;  // TGV: u, ubar, v, vbar, p, q, f; ROF: u, ubar, p, f
+  return planes * size_t(tx + 2 * k) * size_t(ty + 2 * k) * e;
 }
 
-// Workgroup size of a blocked ROF launch: about one batch of pixels per
-// thread of the loaded tile, 256 to 1024 threads.
-static int tv_block_threads(int tx, int ty, int k) {
+// Workgroup size of a blocked launch: about one batch of pixels per thread of
+// the loaded tile (ROF: kTvBlockBatch pixels, TGV: one), 256 to 1024 threads.
+static int block_threads(int kind, int tx, int ty, int k) {
   const int px = (tx + 2 * k) * (ty + 2 * k);
+  if (kind == K_TGV) return px >= 1024 ? 1024 : px >= 384 ? 512 : 256;
   return px > 2048 * kTvBlockBatch / 2 ? 1024 : px > 256 * kTvBlockBatch ? 512 : 256;
 }
 
-void run_tv_block(const Launch& L, int dtype, int dir) {
+void run_block(const Launch& L, int kind, int dtype, bool fp16, int dir) {
   const int tx = L.tb_x, ty = L.tb_y, k = L.tb_k;
-  const size_t lds = tv_block_lds(dtype, tx, ty, k);
+  const size_t lds = block_lds(kind, dtype, tx, ty, k);
   const int grid = ((L.tb_w + tx - 1) / tx) * ((L.tb_h + ty - 1) / ty);
   auto launch = [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
+#define PDT_BLOCK(P, S)                                                                                             \
+  if (kind == K_TGV) hipLaunchKernelGGL((k_tgv_block<P, S, NT>), dim3(grid), dim3(NT), lds, L.stream, L.sl, tx, ty, k, dir); \
+  else hipLaunchKernelGGL((k_tv_block<P, NT>), dim3(grid), dim3(NT), lds, L.stream, L.sl, tx, ty, k, dir)
     switch (dtype) {
-      case DT_F32: hipLaunchKernelGGL((k_tv_block<float, NT>), dim3(grid), dim3(NT), lds, L.stream, L.sl, tx, ty, k, dir); break;
-      case DT_C64: hipLaunchKernelGGL((k_tv_block<Cx<float>, NT>), dim3(grid), dim3(NT), lds, L.stream, L.sl, tx, ty, k, dir); break;
-      case DT_F64: hipLaunchKernelGGL((k_tv_block<double, NT>), dim3(grid), dim3(NT), lds, L.stream, L.sl, tx, ty, k, dir); break;
-      case DT_C128: hipLaunchKernelGGL((k_tv_block<Cx<double>, NT>), dim3(grid), dim3(NT), lds, L.stream, L.sl, tx, ty, k, dir); break;
+      case DT_F32:
+        if (fp16 && kind == K_TGV) hipLaunchKernelGGL((k_tgv_block<float, _Float16, NT>), dim3(grid), dim3(NT), lds, L.stream, L.sl, tx, ty, k, dir);
+        else PDT_BLOCK(float, float);
+        break;
+      case DT_C64:
+        if (fp16 && kind == K_TGV) hipLaunchKernelGGL((k_tgv_block<Cx<float>, Cx<_Float16>, NT>), dim3(grid), dim3(NT), lds, L.stream, L.sl, tx, ty, k, dir);
+        else PDT_BLOCK(Cx<float>, Cx<float>);
+        break;
+      case DT_F64: PDT_BLOCK(double, double); break;
+      case DT_C128: PDT_BLOCK(Cx<double>, Cx<double>); break;
     }
+#undef PDT_BLOCK
   };
-  const int nt = tv_block_threads(tx, ty, k);
+  const int nt = block_threads(kind, tx, ty, k);
   if (nt == 1024) launch(std::integral_constant<int, 1024>{});
   else if (nt == 512) launch(std::integral_constant<int, 512>{});
   else launch(std::integral_constant<int, 256>{});
@@ -2510,6 +2691,20 @@ static bool set_lds_attrs(int b) {
   set((const void*)&k_tv_block<Cx<R>, 512>);
   set((const void*)&k_tv_block<R, 1024>);
   set((const void*)&k_tv_block<Cx<R>, 1024>);
+  set((const void*)&k_tgv_block<R, R, 256>);
+  set((const void*)&k_tgv_block<Cx<R>, Cx<R>, 256>);
+  set((const void*)&k_tgv_block<R, R, 512>);
+  set((const void*)&k_tgv_block<Cx<R>, Cx<R>, 512>);
+  set((const void*)&k_tgv_block<R, R, 1024>);
+  set((const void*)&k_tgv_block<Cx<R>, Cx<R>, 1024>);
+  if constexpr (std::is_same<R, float>::value) {
+    set((const void*)&k_tgv_block<float, _Float16, 256>);
+    set((const void*)&k_tgv_block<Cx<float>, Cx<_Float16>, 256>);
+    set((const void*)&k_tgv_block<float, _Float16, 512>);
+    set((const void*)&k_tgv_block<Cx<float>, Cx<_Float16>, 512>);
+    set((const void*)&k_tgv_block<float, _Float16, 1024>);
+    set((const void*)&k_tgv_block<Cx<float>, Cx<_Float16>, 1024>);
+  }
   with_fft_len<R>(0, one);
   for (int n : kFftLengths)
     if (std::is_same<R, float>::value) with_fft_len<R>(n, one);

@@ -63,8 +63,8 @@ struct Launch {
   // launch asks for enough LDS that no more than that many workgroups fit
   // the LDS of a CU (the autotuner's waves-per-CU axis)
   int wpc[4] = {0, 0, 0, 0};
-  // temporally blocked ROF launches (one dense slice): interior tile tb_x x
-  // tb_y pixels, tb_k iterations per launch (0: off)
+  // temporally blocked denoising launches (one dense ROF or TGV slice):
+  // interior tile tb_x x tb_y pixels, tb_k iterations per launch (0: off)
   int tb_x = 0, tb_y = 0, tb_k = 0;
   int tb_w = 0, tb_h = 0;  // the slice's image size (grid of tiles)
 };
@@ -86,12 +86,13 @@ void run_iteration(const Launch& L, int kind, int dtype, bool fp16, bool gap);
 int num_stages(int kind);
 const char* stage_name(int kind, int stage);
 void run_stage(const Launch& L, int kind, int dtype, bool fp16, bool gap, int stage);
-// tb_k ROF iterations of a single dense slice in one launch (tiles held in
-// LDS with a tb_k-pixel halo), reading u, ubar, p and writing u2, ub2, p2
-// (dir = 0) or the reverse (dir = 1).
-void run_tv_block(const Launch& L, int dtype, int dir);
+// tb_k ROF or TGV (kind) iterations of a single dense slice in one launch
+// (tiles held in LDS with a tb_k-pixel halo), reading the state (ROF: u,
+// ubar, p; TGV: u, v, p, q) and writing its second copy with ubar (and vbar)
+// (dir = 0) or the reverse (dir = 1); fp16: TGV duals stored as half.
+void run_block(const Launch& L, int kind, int dtype, bool fp16, int dir);
 // LDS bytes of that launch, and whether a tile shape fits.
-size_t tv_block_lds(int dtype, int tx, int ty, int k);
+size_t block_lds(int kind, int dtype, int tx, int ty, int k);
 // Gap evaluation from the partial sums of the last iteration (one block/slice).
 void run_gap_reduce(const Launch& L, int kind, int dtype);
 // Operator application in operator mode (op_in -> op_out of every slice).
