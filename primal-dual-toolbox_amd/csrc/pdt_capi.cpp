@@ -896,8 +896,8 @@ struct Engine {
                       : tv.tb[2] >= 0  ? tv.tb
                       : kind == PDT_SOLVER_TGV ? kDefaultTgvBlock
                                                : kDefaultTvBlock;
-      if ((kind == PDT_SOLVER_TV || kind == PDT_SOLVER_TGV) && ns == 1 && tb[2] > 0 && tb[0] > 0 && tb[1] > 0 &&
-          block_lds(kind, dt, tb[0], tb[1], tb[2]) <= init_kernel_attrs()) {
+      if ((kind == PDT_SOLVER_TV || kind == PDT_SOLVER_TGV) && ns == 1 &&
+          block_fits(kind, dt, tb[0], tb[1], tb[2], init_kernel_attrs())) {
         L.tb_x = tb[0];
         L.tb_y = tb[1];
         L.tb_k = tb[2];
@@ -2178,11 +2178,11 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
                  {16, 16, 4}, {16, 16, 6}, {32, 16, 4}, {32, 16, 8}, {32, 32, 8}, {64, 32, 8}, {32, 32, 12},
                  {64, 64, 8}, {64, 64, 16}})
           cand.push_back({{0, 0, 1, 0}, tb, 0});
-      // TGV holds 12 planes per tile pixel: smaller tiles and halos
+      // TGV holds 8 planes per tile pixel in LDS: smaller tiles and halos
       if (sv[0]->kind == PDT_SOLVER_TGV && sv.size() == 1)
         for (const auto& tb : std::vector<std::array<int, 3>>{
                  {16, 8, 2}, {16, 16, 2}, {16, 16, 4}, {32, 8, 2}, {32, 16, 2}, {32, 16, 4}, {32, 32, 2}, {32, 32, 4},
-                 {64, 16, 4}, {64, 32, 3}})
+                 {32, 32, 6}, {48, 32, 4}, {64, 16, 4}, {64, 32, 3}})
           cand.push_back({{0, 0, 1, 0}, tb, 0});
     }
     auto engine = [&](const Cand& c, const int (&wpc)[4]) {

@@ -53,6 +53,14 @@ template <class R> __device__ __forceinline__ Cx<R> operator*(Cx<R> a, Cx<R> b) 
 template <class R> __device__ __forceinline__ Cx<R> operator*(R s, Cx<R> a) { return {s * a.re, s * a.im}; }
 template <class R> __device__ __forceinline__ Cx<R> operator*(Cx<R> a, R s) { return {s * a.re, s * a.im}; }
 template <class R> __device__ __forceinline__ Cx<R> operator/(Cx<R> a, R s) { return {a.re / s, a.im / s}; }
+// a * x + y with one rounding per component.  The denoising primal steps
+// write div + lambda f with it, so that every kernel rounds it the same way
+// (left to the compiler, a kernel that holds f across iterations computed
+// lambda f once, rounded, and the one-iteration kernels fused it).
+template <class R> __device__ __forceinline__ R fmadd(R a, R x, R y) { return fma(a, x, y); }
+template <class R> __device__ __forceinline__ Cx<R> fmadd(R a, Cx<R> x, Cx<R> y) {
+  return {fma(a, x.re, y.re), fma(a, x.im, y.im)};
+}
 // c ? a : b, component-wise (a select, not a branch)
 template <class R> __device__ __forceinline__ Cx<R> csel(bool c, Cx<R> a, Cx<R> b) {
   return {c ? a.re : b.re, c ? a.im : b.im};
@@ -898,7 +906,7 @@ __global__ __launch_bounds__(NT) void k_tv_primal(const SliceDev* sl, const int2
     const size_t i = size_t(y) * W + x;
     const P uo = u[i];
     const P div = dmx<P>(px, x, i, W) + dmy<P>(py, y, i, W, H);
-    P un = uo + tau * (div + lambda * f[i]);
+    P un = uo + tau * fmadd(lambda, P(f[i]), div);
     un = un * rcp(R(1) + tau * lambda);
     u[i] = un;
     ub[i] = (R(1) + theta) * un - theta * uo;
@@ -1052,7 +1060,7 @@ __global__ __launch_bounds__(NT) void k_tv_block(const SliceDev* sl, int tx, int
 #pragma unroll
         for (int j = 0; j < B; ++j) {
           if (e0 + j * NT < cnt) {
-            P un = uo[j] + tau * (div[j] + lambda * ff[j]);
+            P un = uo[j] + tau * fmadd(lambda, ff[j], div[j]);
             un = un * rcp(R(1) + tau * lambda);
             sU[l[j]] = un;
             sB[l[j]] = (R(1) + theta) * un - theta * uo[j];
@@ -1092,21 +1100,30 @@ __device__ __forceinline__ P storage_round(P v) {
 // (TV_primal_u_kernel + TGV_primal_v_kernel) then the dual step
 // (TGV_dual_p_kernel + TGV_dual_q_kernel) per launch, in the reference's
 // order (tgvoptimizer.cu:119-135; tgv_kernels.cuh:225-292,568-605,647-686),
-// on a tile held in LDS.  Block (bx, by) owns the interior [x0, x1) x
-// [y0, y1) and loads u, v (2 planes), p (2), q (3) and f on the interior
-// grown by k pixels on every side (clipped to the image).  The primal step at
-// (x, y) needs p, q at x - 1 and y - 1 (backward differences, none at the
-// first column / row), the dual step ubar, vbar at x + 1 and y + 1 (forward
-// differences, none at the last): the region where the values are exact
-// shrinks by one pixel per step on each side that is not an image border,
-// and after k iterations the interior is exact.  It is written with ubar and
-// vbar to the other copy of the state (src -> dst).  The per-pixel arithmetic
-// is that of k_tgv_primal / k_tgv_dual in the same order, and the duals are
-// rounded to their storage type after every dual step (as a store and a load
-// would round them), so the iterates equal theirs bit for bit.
+// on a tile.  Block (bx, by) owns the interior [x0, x1) x [y0, y1) and loads
+// u, v (2 planes), p (2), q (3) and f on the interior grown by k pixels on
+// every side (clipped to the image).  The primal step at (x, y) needs p, q at
+// x - 1 and y - 1 (backward differences, none at the first column / row), the
+// dual step ubar, vbar at x + 1 and y + 1 (forward differences, none at the
+// last): the region where the values are exact shrinks by one pixel per step
+// on each side that is not an image border, and after k iterations the
+// interior is exact.  It is written with ubar and vbar to the other copy of
+// the state (src -> dst).  Each thread owns up to tgv_block_own pixels of the
+// tile for the whole launch and keeps their u, v and f, which only the
+// pixel itself reads, in registers; ubar, vbar, p and q, which neighbours
+// read, live in LDS (two pixels per thread for 16-byte values, in
+// workgroups of at most 512 threads, whose registers would not hold more).
+// The per-pixel arithmetic
+// is that of k_tgv_primal /
+// k_tgv_dual in the same order, and the duals are rounded to their storage
+// type after every dual step (as a store and a load would round them), so
+// the iterates equal theirs bit for bit.
+constexpr int tgv_block_own(size_t value_bytes) { return value_bytes > 8 ? 2 : 3; }
+
 template <class P, class S, int NT>
 __global__ __launch_bounds__(NT) void k_tgv_block(const SliceDev* sl, int tx, int ty, int k, int dir) {
   extern __shared__ char smem_raw[];
+  constexpr int J = tgv_block_own(sizeof(P));
   using R = typename RealOf<P>::T;
   using L = Ld<P, S>;
   const SliceDev& d = sl[0];
@@ -1117,20 +1134,19 @@ __global__ __launch_bounds__(NT) void k_tgv_block(const SliceDev* sl, int tx, in
   const int by = b / nbx, bx = b - by * nbx;
   const int x0 = bx * tx, x1 = min(W, x0 + tx), y0 = by * ty, y1 = min(H, y0 + ty);
   const int xa = max(0, x0 - k), xb = min(W, x1 + k), ya = max(0, y0 - k), yb = min(H, y1 + k);
-  const int LW = xb - xa, LN = LW * (yb - ya);
-  P* sU = reinterpret_cast<P*>(smem_raw);
-  P* sUB = sU + LN;
-  P* sVX = sUB + LN;
-  P* sVY = sVX + LN;
-  P* sVBX = sVY + LN;
+  const int LW = xb - xa, LN = LW * (yb - ya);  // LN <= J * NT (block_fits)
+  P* sUB = reinterpret_cast<P*>(smem_raw);
+  P* sVBX = sUB + LN;
   P* sVBY = sVBX + LN;
   P* sPX = sVBY + LN;
   P* sPY = sPX + LN;
   P* sQXX = sPY + LN;
   P* sQYY = sQXX + LN;
   P* sQXY = sQYY + LN;
-  P* sF = sQXY + LN;
   const size_t n = size_t(W) * H;
+  // the owned pixels e = threadIdx.x + j NT of the tile (x = -1: none)
+  P ru[J], rvx[J], rvy[J], rf[J];
+  int rx[J], ry[J];
   {
     const auto gu = gptr<const P>(dir ? d.u2 : d.u);
     const auto gv = gptr<const P>(dir ? d.v2 : d.v);
@@ -1138,21 +1154,26 @@ __global__ __launch_bounds__(NT) void k_tgv_block(const SliceDev* sl, int tx, in
     const auto gq = gptr<const S>(dir ? d.q2 : d.q);
     const auto gf = gptr<const P>(d.f);
     const float rLW = 1.0f / float(LW);
-    for (int e = threadIdx.x; e < LN; e += NT) {
-      const int ly = qdiv(e, LW, rLW), lx = e - ly * LW;
-      const size_t gi = size_t(ya + ly) * W + (xa + lx);
-      const P u = gu[gi], vx = gv[gi], vy = gv[gi + n], f = gf[gi];
-      const P px = L::get(gp, gi), py = L::get(gp, gi + n);
-      const P qxx = L::get(gq, gi), qyy = L::get(gq, gi + n), qxy = L::get(gq, gi + 2 * n);
-      sU[e] = u;
-      sVX[e] = vx;
-      sVY[e] = vy;
-      sF[e] = f;
-      sPX[e] = px;
-      sPY[e] = py;
-      sQXX[e] = qxx;
-      sQYY[e] = qyy;
-      sQXY[e] = qxy;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int e = threadIdx.x + j * NT;
+      rx[j] = -1;
+      ry[j] = -1;
+      if (e < LN) {
+        const int ly = qdiv(e, LW, rLW), lx = e - ly * LW;
+        rx[j] = xa + lx;
+        ry[j] = ya + ly;
+        const size_t gi = size_t(ry[j]) * W + rx[j];
+        ru[j] = gu[gi];
+        rvx[j] = gv[gi];
+        rvy[j] = gv[gi + n];
+        rf[j] = gf[gi];
+        sPX[e] = L::get(gp, gi);
+        sPY[e] = L::get(gp, gi + n);
+        sQXX[e] = L::get(gq, gi);
+        sQYY[e] = L::get(gq, gi + n);
+        sQXY[e] = L::get(gq, gi + 2 * n);
+      }
     }
   }
   __syncthreads();
@@ -1173,25 +1194,24 @@ __global__ __launch_bounds__(NT) void k_tgv_block(const SliceDev* sl, int tx, in
   for (int it = 0; it < k; ++it) {
     // primal step where p, q and their backward neighbours are exact
     const int uxa = pxa == 0 ? 0 : pxa + 1, uxb = pxb, uya = pya == 0 ? 0 : pya + 1, uyb = pyb;
-    {
-      const int rw = uxb - uxa, cnt = rw * (uyb - uya);
-      const float rr = 1.0f / float(max(rw, 1));
-      for (int e = threadIdx.x; e < cnt; e += NT) {
-        const int ry = qdiv(e, rw, rr), x = uxa + (e - ry * rw), y = uya + ry;
-        const int l = (y - ya) * LW + (x - xa);
-        const P uo = sU[l];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int x = rx[j], y = ry[j];
+      if (x >= uxa && x < uxb && y >= uya && y < uyb) {
+        const int l = threadIdx.x + j * NT;
+        const P uo = ru[j];
         const P div = bdx(sPX, l, x) + bdy(sPY, l, y);
-        P un = uo + tau * (div + lambda * sF[l]);
+        P un = uo + tau * fmadd(lambda, rf[j], div);
         un = un * rcp(R(1) + tau * lambda);
-        sU[l] = un;
+        ru[j] = un;
         sUB[l] = (R(1) + theta) * un - theta * uo;
         const P dsx = bdx(sQXX, l, x) + bdy(sQXY, l, y);
         const P dsy = bdy(sQYY, l, y) + bdx(sQXY, l, x);
-        const P ox = sVX[l], oy = sVY[l];
+        const P ox = rvx[j], oy = rvy[j];
         const P nx = ox + tau * (dsx + sPX[l]);
         const P ny = oy + tau * (dsy + sPY[l]);
-        sVX[l] = nx;
-        sVY[l] = ny;
+        rvx[j] = nx;
+        rvy[j] = ny;
         sVBX[l] = (R(1) + theta) * nx - theta * ox;
         sVBY[l] = (R(1) + theta) * ny - theta * oy;
       }
@@ -1202,12 +1222,11 @@ __global__ __launch_bounds__(NT) void k_tgv_block(const SliceDev* sl, int tx, in
     pxb = uxb == W ? W : uxb - 1;
     pya = uya;
     pyb = uyb == H ? H : uyb - 1;
-    {
-      const int rw = pxb - pxa, cnt = rw * (pyb - pya);
-      const float rr = 1.0f / float(max(rw, 1));
-      for (int e = threadIdx.x; e < cnt; e += NT) {
-        const int ry = qdiv(e, rw, rr), x = pxa + (e - ry * rw), y = pya + ry;
-        const int l = (y - ya) * LW + (x - xa);
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int x = rx[j], y = ry[j];
+      if (x >= pxa && x < pxb && y >= pya && y < pyb) {
+        const int l = threadIdx.x + j * NT;
         const int lx = l + (x < W - 1), ly = l + ((y < H - 1) ? LW : 0);
         P px = sPX[l], py = sPY[l], qxx = sQXX[l], qyy = sQYY[l], qxy = sQXY[l];
         dual_math<P, true, false>(sUB[l], sUB[lx], sUB[ly], sVBX[l], sVBY[l], sVBX[lx], sVBY[lx], sVBX[ly], sVBY[ly],
@@ -1227,23 +1246,24 @@ __global__ __launch_bounds__(NT) void k_tgv_block(const SliceDev* sl, int tx, in
   const auto ovb = gptr<P>(dir ? d.vb : d.vb2);
   const auto op = gptr<S>(dir ? d.p : d.p2);
   const auto oq = gptr<S>(dir ? d.q : d.q2);
-  const int iw = x1 - x0, icnt = iw * (y1 - y0);
-  const float riw = 1.0f / float(iw);
-  for (int e = threadIdx.x; e < icnt; e += NT) {
-    const int ry = qdiv(e, iw, riw), x = x0 + (e - ry * iw), y = y0 + ry;
-    const int l = (y - ya) * LW + (x - xa);
-    const size_t gi = size_t(y) * W + x;
-    ou[gi] = sU[l];
-    oub[gi] = sUB[l];
-    ov[gi] = sVX[l];
-    ov[gi + n] = sVY[l];
-    ovb[gi] = sVBX[l];
-    ovb[gi + n] = sVBY[l];
-    L::put(op, gi, sPX[l]);
-    L::put(op, gi + n, sPY[l]);
-    L::put(oq, gi, sQXX[l]);
-    L::put(oq, gi + n, sQYY[l]);
-    L::put(oq, gi + 2 * n, sQXY[l]);
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int x = rx[j], y = ry[j];
+    if (x >= x0 && x < x1 && y >= y0 && y < y1) {
+      const int l = threadIdx.x + j * NT;
+      const size_t gi = size_t(y) * W + x;
+      ou[gi] = ru[j];
+      oub[gi] = sUB[l];
+      ov[gi] = rvx[j];
+      ov[gi + n] = rvy[j];
+      ovb[gi] = sVBX[l];
+      ovb[gi + n] = sVBY[l];
+      L::put(op, gi, sPX[l]);
+      L::put(op, gi + n, sPY[l]);
+      L::put(oq, gi, sQXX[l]);
+      L::put(oq, gi + n, sQYY[l]);
+      L::put(oq, gi + 2 * n, sQXY[l]);
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += k;
 }
@@ -1273,7 +1293,7 @@ __global__ __launch_bounds__(NT) void k_tgv_primal(const SliceDev* sl, const int
     const size_t i = size_t(y) * W + x;
     const P uo = u[i];
     const P div = dmx<P>(px, x, i, W) + dmy<P>(py, y, i, W, H);
-    P un = uo + tau * (div + lambda * f[i]);
+    P un = uo + tau * fmadd(lambda, P(f[i]), div);
     un = un * rcp(R(1) + tau * lambda);
     u[i] = un;
     ub[i] = (R(1) + theta) * un - theta * uo;
@@ -2459,16 +2479,26 @@ static void with_fft_len(int n, Fn&& fn) {
 
 size_t block_lds(int kind, int dtype, int tx, int ty, int k) {
   const size_t e = dtype == DT_F32 ? 4 : (dtype == DT_C128 ? 16 : 8);
-  const size_t planes = kind == K_TGV ? 12 : 5;  // TGV: u, ubar, v, vbar, p, q, f; ROF: u, ubar, p, f
+  const size_t planes = kind == K_TGV ? 8 : 5;  // TGV: ubar, vbar, p, q; ROF: u, ubar, p, f
   return planes * size_t(tx + 2 * k) * size_t(ty + 2 * k) * e;
 }
 
 // Workgroup size of a blocked launch: about one batch of pixels per thread of
-// the loaded tile (ROF: kTvBlockBatch pixels, TGV: one), 256 to 1024 threads.
-static int block_threads(int kind, int tx, int ty, int k) {
+// the loaded tile (ROF: kTvBlockBatch pixels, TGV: one to tgv_block_own),
+// 256 to 1024 threads.
+static int block_threads(int kind, int dtype, int tx, int ty, int k) {
   const int px = (tx + 2 * k) * (ty + 2 * k);
-  if (kind == K_TGV) return px >= 1024 ? 1024 : px >= 384 ? 512 : 256;
+  if (kind == K_TGV) return px >= 1024 && dtype != DT_C128 ? 1024 : px >= 384 ? 512 : 256;
   return px > 2048 * kTvBlockBatch / 2 ? 1024 : px > 256 * kTvBlockBatch ? 512 : 256;
+}
+
+bool block_fits(int kind, int dtype, int tx, int ty, int k, size_t lds_limit) {
+  if (tx <= 0 || ty <= 0 || k <= 0) return false;
+  const int px = (tx + 2 * k) * (ty + 2 * k);
+  // TGV: every tile pixel owned by a thread (tgv_block_own per thread at most)
+  const size_t e = dtype == DT_F32 ? 4 : (dtype == DT_C128 ? 16 : 8);
+  if (kind == K_TGV && px > tgv_block_own(e) * block_threads(kind, dtype, tx, ty, k)) return false;
+  return block_lds(kind, dtype, tx, ty, k) <= lds_limit;
 }
 
 void run_block(const Launch& L, int kind, int dtype, bool fp16, int dir) {
@@ -2494,7 +2524,7 @@ void run_block(const Launch& L, int kind, int dtype, bool fp16, int dir) {
     }
 #undef PDT_BLOCK
   };
-  const int nt = block_threads(kind, tx, ty, k);
+  const int nt = block_threads(kind, dtype, tx, ty, k);
   if (nt == 1024) launch(std::integral_constant<int, 1024>{});
   else if (nt == 512) launch(std::integral_constant<int, 512>{});
   else launch(std::integral_constant<int, 256>{});

@@ -91,8 +91,10 @@ void run_stage(const Launch& L, int kind, int dtype, bool fp16, bool gap, int st
 // ubar, p; TGV: u, v, p, q) and writing its second copy with ubar (and vbar)
 // (dir = 0) or the reverse (dir = 1); fp16: TGV duals stored as half.
 void run_block(const Launch& L, int kind, int dtype, bool fp16, int dir);
-// LDS bytes of that launch, and whether a tile shape fits.
+// LDS bytes of that launch, and whether a tile shape fits (the LDS a
+// workgroup may take, and for TGV the pixels its threads can own).
 size_t block_lds(int kind, int dtype, int tx, int ty, int k);
+bool block_fits(int kind, int dtype, int tx, int ty, int k, size_t lds_limit);
 // Gap evaluation from the partial sums of the last iteration (one block/slice).
 void run_gap_reduce(const Launch& L, int kind, int dtype);
 // Operator application in operator mode (op_in -> op_out of every slice).
