@@ -1955,6 +1955,41 @@ int pdt_solver_destroy(pdt_solver* s) {
   return guard([&] { delete s; });
 }
 
+// The solvers' iterates (u, ubar, v, vbar, p, q and the MRI dual r), copied
+// aside while the autotuner or the kernel profile runs iterations on them
+// and put back when it ends: a solve after either starts where it would
+// have without it.
+class StateGuard {
+ public:
+  explicit StateGuard(const std::vector<pdt_solver*>& sv) {
+    HIPC(hipDeviceSynchronize());
+    for (auto* s : sv) {
+      if (s->u) keep(s->u->ptr, s->u->bytes());
+      if (s->v) keep(s->v->ptr, s->v->bytes());
+      keep(s->ub.p, s->ub.bytes);
+      keep(s->vb.p, s->vb.bytes);
+      keep(s->p.p, s->p.bytes);
+      keep(s->q.p, s->q.bytes);
+      keep(s->rk.p, s->rk.bytes);
+    }
+  }
+  ~StateGuard() {
+    (void)hipDeviceSynchronize();
+    for (auto& k : saved_) (void)hipMemcpy(k.first, k.second.p, k.second.bytes, hipMemcpyDeviceToDevice);
+  }
+  StateGuard(const StateGuard&) = delete;
+  StateGuard& operator=(const StateGuard&) = delete;
+
+ private:
+  void keep(void* ptr, size_t bytes) {
+    if (!ptr || !bytes) return;
+    DevBuf b(bytes);
+    HIPC(hipMemcpy(b.p, ptr, bytes, hipMemcpyDeviceToDevice));
+    saved_.emplace_back(ptr, std::move(b));
+  }
+  std::vector<std::pair<void*, DevBuf>> saved_;
+};
+
 namespace {
 // Per-stage duration and algorithmic bytes of one launch, over a solver or a
 // batch (pdt_solver_profile, pdt_batch_profile).
@@ -1962,6 +1997,7 @@ void profile_engine(std::vector<pdt_solver*> sv, int reps, pdt_kernel_prof* out,
   if (!out || !n || reps <= 0) fail(PDT_ERR_ARG, "bad argument");
   check_batch(sv);
   for (auto* s : sv) prepare_solve(s, 0);
+  StateGuard keep_state(sv);
   {
     Engine e;
     e.sv = sv;
@@ -2144,6 +2180,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
     std::vector<pdt_solver*> sv(solvers, solvers + count);
     check_batch(sv);
     for (auto* s : sv) prepare_solve(s, 0);
+    StateGuard keep_state(sv);
     const bool mri = sv[0]->is_op();
     // Phase 1, tile shapes without occupancy caps: coils per forward row
     // item, columns per column item, rows per denoising item, coils per
