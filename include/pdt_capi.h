@@ -196,9 +196,8 @@ PDT_API int pdt_solver_destroy(pdt_solver* s);
  * bytes of one launch (every distinct element the launch reads or writes,
  * once, at storage precision).  Stages that launch nothing for this solver
  * (the separate primal kernel when every slice has one coil group) are
- * omitted; *n receives the number of entries written.  Starts from the
- * solver's current state and leaves it advanced: call it after the solves
- * whose results are used. */
+ * omitted; *n receives the number of entries written.  Runs from the
+ * solver's current state and puts its iterates back at the end. */
 typedef struct pdt_kernel_prof {
   char name[32];
   double ms;
@@ -220,29 +219,32 @@ typedef struct pdt_tiles {
   int32_t cb_a;    /* coils per LDS chunk of the adjoint row pass (MRI)          */
   int32_t wpc[4];  /* workgroups per CU, 0 = no cap: forward row, column,
                       adjoint row and denoising passes                          */
-  int32_t tb_x, tb_y, tb_k; /* ROF, one slice: tb_k iterations per launch on
-                      tb_x x tb_y tiles held in LDS (tb_k 0: one iteration per
-                      pair of launches)                                          */
+  int32_t tb_x, tb_y, tb_k; /* ROF or TGV denoising, one slice: tb_k
+                      iterations per launch on tb_x x tb_y tiles held in LDS
+                      (tb_k 0: one iteration per pair of launches)               */
   int32_t dual_px; /* TGV denoising: pixels per thread of the dual step, 1 or 2
                       (2 needs even widths; 0: the default, 2 where possible)   */
 } pdt_tiles;
 
 /* NEW: autotuner.  Times every candidate tile shape (the fields of
- * pdt_tiles; for a single ROF slice also temporally blocked launches) on the
+ * pdt_tiles; for a single ROF or TGV denoising slice also temporally blocked
+ * launches) on the
  * configured solvers (one batch) with hipEvents, per iteration; then, with
  * those tiles, caps of 1, 2, 3, 4 or 6 workgroups per CU (3, 4 or 6 for
  * denoising) on each pass in turn, each pass timed alone, keeping a cap only
  * where it beats no cap by 2 %.  Stores the choice in the tile cache and
- * returns it (out may be NULL) with the time per iteration it gives.
- * Advances the solver state: call before the solves and reset with
- * set_input0. */
+ * returns it (out may be NULL) with the time per iteration it gives.  The
+ * timing runs iterate on the solvers' buffers; their iterates are copied
+ * aside first and put back at the end, so a solve after tuning starts where
+ * it would have without it. */
 PDT_API int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, double* ms_per_iteration);
 
 /* NEW: a manual tile choice for the shape of these solvers (one batch),
  * stored in the same tile cache as pdt_autotune's result: cb, cg, rpi, cb_a
  * <= 0 take the default, values above a shape's limit are clamped; wpc in
- * [0, 6]; tb_k <= 0 turns the blocked ROF launches off, and tiles that do
- * not fit the LDS are not used.  The next solve of the same shape uses it. */
+ * [0, 6]; tb_k <= 0 turns the blocked denoising launches off, and tiles
+ * that do not fit the LDS are not used.  The next solve of the same shape
+ * uses it. */
 PDT_API int pdt_set_tiles(pdt_solver** solvers, int count, const pdt_tiles* tiles);
 
 /* NEW: solve many configured solvers (same kind and dtype) together, with one

@@ -236,8 +236,7 @@ class _Optimizer:
         iteration loop (the loop time per iteration, from hipEvents around a
         graph of reps iterations, divided in proportion to event pairs around
         each kernel's direct launches) and algorithmic bytes per launch.
-        Advances the state arbitrarily, so call it after the solves whose
-        results are used."""
+        The iterates are put back after the timing runs."""
         self._push_params()
         out = (C.PdtKernelProf * 8)()
         n = ct.c_int()
@@ -256,11 +255,11 @@ def _tiles_dict(t) -> dict:
 
 
 def autotune(optimizers, reps: int = 20) -> dict:
-    """Times the candidate tile shapes (for one ROF slice also temporally
-    blocked launches), then occupancy caps (workgroups per CU) per pass, on
-    these configured optimizers (one batch), caches the fastest and returns
-    it.  Leaves the state advanced: reset with setInput0 / setNoisyData
-    before solving."""
+    """Times the candidate tile shapes (for one ROF or TGV denoising slice
+    also temporally blocked launches), then occupancy caps (workgroups per
+    CU) per pass, on these configured optimizers (one batch), caches the
+    fastest and returns it.  The optimizers' iterates are put back after
+    the timing runs, so a solve after tuning starts where it would have."""
     for o in optimizers:
         o._push_params()
     arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
@@ -277,7 +276,8 @@ def set_tiles(optimizers, cb: int = 0, cg: int = 0, rpi: int = 0, cb_a: int = 0,
     denoising item (rpi), 0 = the default; wg_per_cu: 4 occupancy caps
     (forward row, column, adjoint row, denoising passes; 0 = none);
     tv_block: (tile x, tile y, iterations per launch) of the temporally
-    blocked ROF launches, None or iterations 0 = off; dual_px: pixels per
+    blocked ROF / TGV denoising launches, None or iterations 0 = off
+    (the one-iteration kernels); dual_px: pixels per
     thread of the TGV denoising dual step (1, 2; 0 = the default)."""
     for o in optimizers:
         o._push_params()
@@ -290,7 +290,8 @@ def set_tiles(optimizers, cb: int = 0, cg: int = 0, rpi: int = 0, cb_a: int = 0,
 def batch_profile(optimizers, reps: int = 20) -> list:
     """profileKernels over the launches of a batch of configured optimizers:
     per fused kernel, ms per launch over every slice and the algorithmic bytes
-    of that launch summed over the slices.  Advances the states."""
+    of that launch summed over the slices.  The iterates are put back after
+    the timing runs."""
     for o in optimizers:
         o._push_params()
     arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
