@@ -195,18 +195,21 @@ def _batch_pair(pdt, specs):
     return out
 
 
-@pytest.mark.parametrize("solver,coils", [("tgvmri", 8), ("tgvmri", 15), ("tvop", 15)])
-def test_single_slice_step_workgroups_equal_batch(solver, coils):
+@pytest.mark.parametrize("solver,coils,n", [("tgvmri", 8, 320), ("tgvmri", 15, 320), ("tvop", 15, 320),
+                                            ("tgvmri", 20, 256)])
+def test_single_slice_step_workgroups_equal_batch(solver, coils, n):
     """A single-slice solve runs the dual step and, with wide adjoint blocks,
     the v step in workgroups of their own, and its adjoint row blocks take
     the straight-line path (sensitivity loads in flight during the FFT); a
     batch runs both steps inside the row workgroups and the general adjoint
-    path.  320 x 320, 8 or 15 coils, TGV-MRI and TV with the operator, the
-    same tiles (all coils of a row in one adjoint block) for both: the
-    iterates agree bit for bit after 100 iterations, the gap to 1e-12."""
+    path.  320 x 320 with 8 or 15 coils (three thread groups of up to five
+    coil products per adjoint block) and 256 x 256 with 20 coils (four
+    groups of five), TGV-MRI and TV with the operator, the same tiles (all
+    coils of a row in one adjoint block) for both: the iterates agree bit for
+    bit after 100 iterations, the gap to 1e-12."""
     pdt = _pdt()
     from primaldualtoolbox import synthetic as syn
-    case = syn.mri_case(seed=31, n=320, coils=coils)
+    case = syn.mri_case(seed=31, n=n, coils=coils)
     a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
     cls = {"tgvmri": pdt.mri.TgvMriOptimizer_2c3c, "tvop": pdt.mri.TvOptimizerWithOp_2c3c}[solver]
     prm = dict(gap_every=50) if solver == "tgvmri" else dict(gap_every=50, Lambda=0.5)
