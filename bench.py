@@ -159,6 +159,7 @@ def held_out_end_states(gold: str) -> list:
     """The held-out slices that have FP64 oracle end states in the fixture
     file (the network never saw them): cold and warm time-to-gap, iterations,
     and rel. L2 of each GPU end state vs the oracle's u at its own stop."""
+    from primaldualtoolbox import _capi as C
     from primaldualtoolbox import synthetic as syn
     from primaldualtoolbox import warmstart as ws
     if not os.path.exists(ws.DEFAULT_WEIGHTS) or not os.path.exists(gold):
@@ -170,7 +171,11 @@ def held_out_end_states(gold: str) -> list:
         for seed in seeds:
             case = syn.mri320_case(seed)
             op, opt = build_solver(case, float(z[f"alpha1_{seed}"]))
-            u0 = op.adjoint(case["kspace"])
+            # u0 = A^* f on the device, as for the headline's cold start
+            f_dev = C.DeviceArray.from_numpy(case["kspace"], np.complex64)
+            H, W = case["image"].shape
+            u0_dev = C.DeviceArray.empty(C.PDT_C64, [W, H])
+            C.check(C.lib().pdt_op_adjoint(op._h, f_dev.h, u0_dev.h))
             rec = {"seed": seed, "oracle_iterations": int(z[f"iters_{seed}"])}
             for start in ("cold", "warm"):
                 best = None
@@ -180,7 +185,7 @@ def held_out_end_states(gold: str) -> list:
                     if start == "warm":
                         net.warm_start(opt)
                     else:
-                        opt.setInput0(u0)
+                        cold_start(opt, u0_dev)
                     opt.solve()
                     dt = time.perf_counter() - t0
                     best = dt if best is None else min(best, dt)
