@@ -224,6 +224,11 @@ typedef struct pdt_tiles {
                       (tb_k 0: one iteration per pair of launches)               */
   int32_t dual_px; /* TGV denoising: pixels per thread of the dual step, 1 or 2
                       (2 needs even widths; 0: the default, 2 where possible)   */
+  int32_t unpaired; /* MRI, one slice: 0 (default) = an iteration without the
+                      gap evaluation in two launches where the adjoint row pass
+                      holds every coil (the column pass also does the dual
+                      step, the adjoint row pass also the next iteration's
+                      forward row FFTs); 1 = always the three passes         */
 } pdt_tiles;
 
 /* NEW: autotuner.  Times every candidate tile shape (the fields of

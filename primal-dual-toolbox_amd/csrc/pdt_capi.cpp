@@ -804,6 +804,8 @@ struct Engine {
   int force_cb = 0, force_cg = 0, force_rpi = 0, force_cb_a = 0;  // autotuner overrides (0: cache / default)
   int force_wpc[4] = {-1, -1, -1, -1};  // occupancy caps (-1: cache; 0: none)
   int force_tb[3] = {0, 0, -1};         // blocked ROF tiles (tb[2] -1: cache / default; 0: off)
+  int force_unpaired = -1;              // MRI: 1 three launches per iteration, 0 pairs where possible (-1: cache)
+  bool unpaired = false;
   bool plan_only = false;                         // build(): stop after the signature
   std::string sig;
   TuneKey key{};
@@ -872,6 +874,7 @@ struct Engine {
       key = TuneKey{kind, dt, maxW, sv[0]->H, maxHk, maxC, maxNc, ns};
       TuneVal tv = tune_lookup(key);
       for (int i = 0; i < 4; ++i) L.wpc[i] = force_wpc[i] >= 0 ? force_wpc[i] : tv.wpc[i];
+      unpaired = force_unpaired >= 0 ? force_unpaired != 0 : tv.unpaired != 0;
       cb = force_cb > 0 ? force_cb : tv.cb > 0 ? tv.cb : kDefaultCb;
       cg = force_cg > 0 ? force_cg : tv.cg > 0 ? tv.cg : kDefaultCg;
       cb_a = force_cb_a > 0 ? force_cb_a : tv.cb_a > 0 ? tv.cb_a : cb;
@@ -1005,9 +1008,9 @@ struct Engine {
     // signature of everything the kernels and graphs depend on (the control
     // and statistics words are owned by the engine and added below)
     sig.assign(reinterpret_cast<const char*>(hs.data()), hs.size() * sizeof(SliceDev));
-    const int tiles[13] = {cb,       cg,       rpi,      int(sv[0]->prm.gap_every), cb_a,   L.wpc[0],
+    const int tiles[14] = {cb,       cg,       rpi,      int(sv[0]->prm.gap_every), cb_a,   L.wpc[0],
                            L.wpc[1], L.wpc[2], L.wpc[3], L.tb_x,                    L.tb_y, L.tb_k,
-                           int(pair_duals)};
+                           int(pair_duals), int(unpaired)};
     sig.append(reinterpret_cast<const char*>(tiles), sizeof(tiles));
     if (plan_only) return;
     cs_bytes = size_t(ns) * (ST_WORDS * sizeof(double) + CTL_WORDS * sizeof(int));
@@ -1187,6 +1190,7 @@ struct Engine {
         const SliceDev& d0 = hs[groups[r.g][0]];
         x.adj_fused = dense && x.adj_threads == kRowAdjThreadsWide && d0.ngr_a == 1 && d0.S != nullptr &&
                       rows_adj_fused_ok(dt, d0.W, d0.C, d0.ncols) && (!tgv || nvstep > 0);
+        x.pair = x.adj_fused && ndual > 0 && dt == DT_C64 && !unpaired;
       }
       if (!(dense && (kind == PDT_SOLVER_TV || kind == PDT_SOLVER_TGV))) x.tb_x = x.tb_y = x.tb_k = 0;
       x.tb_w = hs[groups[r.g][0]].W;
@@ -1275,6 +1279,35 @@ struct Engine {
   }
 
   bool blocked() const { return G.size() == 1 && G[0].tb_k > 0; }
+  bool paired() const { return G.size() == 1 && G[0].pair; }
+
+  // The launches of one iteration without the gap evaluation, as timing and
+  // profiling take them: the two of a paired iteration (the column pass with
+  // the dual step, the adjoint row pass with the next forward row FFTs) or
+  // the stages.
+  int iter_launches() const { return paired() ? 2 : num_stages(kind); }
+  // before timing the launches alone: the T1 the first column pass reads
+  void iter_prologue() {
+    if (paired()) stage(0, false);
+  }
+  void iter_launch(int st) {
+    if (paired()) run_pair_stage(G[0], kind, dtk(), st);
+    else stage(st, false);
+  }
+  std::string iter_launch_name(int st) const {
+    if (paired()) return st == 0 ? "k_cols_dual" : "k_rows_adj_next";
+    return stage_name(kind, st);
+  }
+  // algorithmic bytes: the column pass plus the dual step's planes, the
+  // adjoint row pass plus the T1 it writes
+  double iter_launch_bytes(int st) const {
+    if (!paired()) return stage_bytes(st);
+    const pdt_solver* s = sv[0];
+    const SliceDev& d = hs[0];
+    const double N = double(s->W) * s->H, pe = double(s->pe());
+    const bool tgv = kind != PDT_SOLVER_TV_OP;
+    return st == 0 ? stage_bytes(1) + (tgv ? 13 * pe : 5 * pe) * N : stage_bytes(2) + double(d.C) * d.H * d.ncols * pe;
+  }
 
   // Enqueues `len` iterations; the last one evaluates the gap when asked.
   // With temporally blocked ROF launches, the iterations before the gap
@@ -1290,6 +1323,30 @@ struct Engine {
         run_block(G[0], kind, dt, fp16, 1);
       }
       for (int i = 0; i < plain; ++i) iteration(false);
+      if (gap) {
+        iteration(true);
+        run_gap_reduce(L, kind, dt);
+      }
+      return;
+    }
+    if (paired()) {
+      // iterations before the gap iteration in pairs of launches: the first
+      // takes its T1 from the forward row pass (which also does its dual
+      // step), every other from the adjoint row pass before it; the last
+      // one's adjoint row pass writes no T1, as the next iteration (the gap
+      // iteration, or the first of the next chunk) starts with the forward
+      // row pass again
+      const int plain = len - (gap ? 1 : 0);
+      for (int i = 0; i < plain; ++i) {
+        if (i == 0) {
+          stage(0, false);
+          stage(1, false);
+        } else {
+          run_pair_stage(G[0], kind, dt, 0);
+        }
+        if (i == plain - 1) stage(2, false);
+        else run_pair_stage(G[0], kind, dt, 1);
+      }
       if (gap) {
         iteration(true);
         run_gap_reduce(L, kind, dt);
@@ -2051,13 +2108,17 @@ void profile_engine(std::vector<pdt_solver*> sv, int reps, pdt_kernel_prof* out,
     // stage's share of it comes from event pairs around its direct launches
     // in iteration order (e_st); an event pair adds a fixed dispatch delay to
     // every launch, so the stage duration reported is t_it * e_st / sum(e).
-    const int ns = num_stages(e.kind);
+    // (A paired engine's iterations are taken as the two launches of the
+    // steady state, each reading the T1 the one before wrote.)
+    const int ns = e.iter_launches();
+    e.iter_prologue();
     hipEvent_t e0, e1;
     HIPC(hipEventCreate(&e0));
     HIPC(hipEventCreate(&e1));
     hipGraph_t graph;
     HIPC(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
-    for (int r = 0; r < reps; ++r) e.iteration(false);
+    for (int r = 0; r < reps; ++r)
+      for (int st = 0; st < ns; ++st) e.iter_launch(st);
     HIPC(hipStreamEndCapture(e.stream, &graph));
     hipGraphExec_t ge;
     HIPC(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
@@ -2077,7 +2138,7 @@ void profile_engine(std::vector<pdt_solver*> sv, int reps, pdt_kernel_prof* out,
     for (int r = 0; r < reps; ++r)
       for (int st = 0; st < ns; ++st) {
         HIPC(hipEventRecord(ev[(size_t(r) * ns + st) * 2], e.stream));
-        e.stage(st, false);
+        e.iter_launch(st);
         HIPC(hipEventRecord(ev[(size_t(r) * ns + st) * 2 + 1], e.stream));
       }
     HIPC(hipStreamSynchronize(e.stream));
@@ -2085,7 +2146,7 @@ void profile_engine(std::vector<pdt_solver*> sv, int reps, pdt_kernel_prof* out,
     std::vector<double> share(ns, 0.0);
     double sum = 0;
     for (int st = 0; st < ns; ++st) {
-      if (e.stage_bytes(st) <= 0) continue;  // a stage without launches (primal step inside k_rows_adj)
+      if (e.iter_launch_bytes(st) <= 0) continue;  // a stage without launches (primal step inside k_rows_adj)
       for (int r = 0; r < reps; ++r) {
         float ms = 0;
         HIPC(hipEventElapsedTime(&ms, ev[(size_t(r) * ns + st) * 2], ev[(size_t(r) * ns + st) * 2 + 1]));
@@ -2097,9 +2158,9 @@ void profile_engine(std::vector<pdt_solver*> sv, int reps, pdt_kernel_prof* out,
     const double t_it = double(loop_ms) / reps;
     int k = 0;
     for (int st = 0; st < ns && k < cap; ++st) {
-      const double bytes = e.stage_bytes(st);
+      const double bytes = e.iter_launch_bytes(st);
       if (bytes <= 0) continue;
-      std::snprintf(out[k].name, sizeof(out[k].name), "%s", stage_name(e.kind, st));
+      std::snprintf(out[k].name, sizeof(out[k].name), "%s", e.iter_launch_name(st).c_str());
       out[k].ms = sum > 0 ? t_it * share[st] / sum : 0.0;
       out[k].bytes = bytes;
       ++k;
@@ -2131,10 +2192,11 @@ std::vector<double> time_stages(Engine& e, int reps) {
   HIPC(hipEventCreate(&e0));
   HIPC(hipEventCreate(&e1));
   std::vector<double> out;
-  for (int st = 0; st < num_stages(e.kind); ++st) {
-    e.stage(st, false);
+  e.iter_prologue();
+  for (int st = 0; st < e.iter_launches(); ++st) {
+    e.iter_launch(st);
     HIPC(hipEventRecord(e0, e.stream));
-    for (int r = 0; r < reps; ++r) e.stage(st, false);
+    for (int r = 0; r < reps; ++r) e.iter_launch(st);
     HIPC(hipEventRecord(e1, e.stream));
     HIPC(hipEventSynchronize(e1));
     HIPC(hipGetLastError());
@@ -2192,6 +2254,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
       std::array<int, 4> t;
       std::array<int, 3> tb;
       int dpx = 0;
+      int unp = 0;  // MRI: three launches per iteration where two would do
     };
     std::vector<Cand> cand;
     if (mri) {
@@ -2205,6 +2268,8 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
           for (int cg : {1, 2, 4}) {
             cand.push_back({{cb, cg, 0, cb}, {0, 0, 0}, 0});
             if (cb != maxC) cand.push_back({{cb, cg, 0, maxC}, {0, 0, 0}, 0});
+            // every coil in the adjoint row blocks: also the three-launch iteration
+            cand.push_back({{cb, cg, 0, maxC}, {0, 0, 0}, 0, 1});
           }
     } else {
       for (int rpi : {1, 2, 4, 8})
@@ -2231,6 +2296,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
       e->force_cb_a = c.t[3];
       for (int i = 0; i < 3; ++i) e->force_tb[i] = c.tb[i];
       e->force_dpx = c.dpx;
+      e->force_unpaired = c.unp;
       for (int i = 0; i < 4; ++i) e->force_wpc[i] = wpc[i];
       e->build();
       e->reset_ctl();
@@ -2238,8 +2304,9 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
     };
     const int none[4] = {0, 0, 0, 0};
     double best = 1e300;
-    Cand bc{{0, 0, 0, 0}, {0, 0, 0}, 0};
+    Cand bc{{0, 0, 0, 0}, {0, 0, 0}, 0, 0};
     std::vector<double> best_st;
+    bool best_paired = false;
     TuneKey key{};
     for (const auto& c : cand) {
       auto e = engine(c, none);
@@ -2250,8 +2317,9 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
       key = e->key;
       if (ms < best) {
         best = ms;
-        bc = {{e->L.cb, e->L.cg, e->L.rpi, e->L.cb_a}, {e->G[0].tb_x, e->G[0].tb_y, e->G[0].tb_k}, c.dpx};
+        bc = {{e->L.cb, e->L.cg, e->L.rpi, e->L.cb_a}, {e->G[0].tb_x, e->G[0].tb_y, e->G[0].tb_k}, c.dpx, c.unp};
         best_st = st;
+        best_paired = e->paired();
       }
     }
     // Phase 2, waves per CU: with the chosen tiles, a cap on the workgroups
@@ -2261,7 +2329,10 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
     // blocked ROF launches take none.
     int wpc[4] = {0, 0, 0, 0};
     std::vector<std::pair<int, std::vector<int>>> axes;  // (cap slot, stages it governs)
-    if (mri) axes = {{WPC_FWD, {0}}, {WPC_COLS, {1}}, {WPC_ADJ, {2}}};
+    // (a paired iteration: the column and adjoint row launches, the forward
+    // row pass running once per chunk only)
+    if (mri && best_paired) axes = {{WPC_COLS, {0}}, {WPC_ADJ, {1}}};
+    else if (mri) axes = {{WPC_FWD, {0}}, {WPC_COLS, {1}}, {WPC_ADJ, {2}}};
     else if (bc.tb[2] == 0) axes = {{WPC_DEN, {0, 1}}};
     const std::vector<int> caps = mri ? std::vector<int>{1, 2, 3, 4, 6} : std::vector<int>{3, 4, 6};
     for (const auto& ax : axes) {
@@ -2287,6 +2358,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
     for (int i = 0; i < 4; ++i) tv.wpc[i] = wpc[i];
     for (int i = 0; i < 3; ++i) tv.tb[i] = bc.tb[i];
     tv.dual_px = bc.dpx;
+    tv.unpaired = bc.unp;
     tune_store(key, tv);
     double ms = 0;
     {
@@ -2303,6 +2375,7 @@ int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, doub
       out->tb_y = bc.tb[1];
       out->tb_k = bc.tb[2];
       out->dual_px = bc.dpx;
+      out->unpaired = bc.unp;
     }
     if (out_ms) *out_ms = ms;
   });
@@ -2327,6 +2400,7 @@ int pdt_set_tiles(pdt_solver** solvers, int count, const pdt_tiles* t) {
     e.force_tb[1] = t->tb_y;
     e.force_tb[2] = t->tb_k > 0 ? t->tb_k : 0;
     e.force_dpx = t->dual_px == 1 || t->dual_px == 2 ? t->dual_px : 0;
+    e.force_unpaired = t->unpaired != 0 ? 1 : 0;
     e.build();
     const bool mri = sv[0]->is_op();
     TuneVal tv = mri ? TuneVal{e.L.cb, e.L.cg, 0, e.L.cb_a} : TuneVal{0, 0, e.L.rpi, 0};
@@ -2335,6 +2409,7 @@ int pdt_set_tiles(pdt_solver** solvers, int count, const pdt_tiles* t) {
     tv.tb[1] = e.L.tb_y;
     tv.tb[2] = e.L.tb_k;
     tv.dual_px = e.force_dpx;
+    tv.unpaired = e.unpaired ? 1 : 0;
     tune_store(e.key, tv);
   });
 }

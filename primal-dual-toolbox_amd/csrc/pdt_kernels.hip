@@ -47,8 +47,12 @@ struct alignas(2 * sizeof(R)) Cx {
 template <class R> __device__ __forceinline__ Cx<R> operator+(Cx<R> a, Cx<R> b) { return {a.re + b.re, a.im + b.im}; }
 template <class R> __device__ __forceinline__ Cx<R> operator-(Cx<R> a, Cx<R> b) { return {a.re - b.re, a.im - b.im}; }
 template <class R> __device__ __forceinline__ Cx<R> operator-(Cx<R> a) { return {-a.re, -a.im}; }
+// Complex products with explicit fused multiply-adds: left to the compiler,
+// the contraction of a.re b.im + a.im b.re fused one product in one kernel
+// and the other in another (both are one rounding short of two), so the same
+// product of the same values differed in the last bit between kernels.
 template <class R> __device__ __forceinline__ Cx<R> operator*(Cx<R> a, Cx<R> b) {
-  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  return {fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)};
 }
 template <class R> __device__ __forceinline__ Cx<R> operator*(R s, Cx<R> a) { return {s * a.re, s * a.im}; }
 template <class R> __device__ __forceinline__ Cx<R> operator*(Cx<R> a, R s) { return {s * a.re, s * a.im}; }
@@ -67,7 +71,7 @@ template <class R> __device__ __forceinline__ Cx<R> csel(bool c, Cx<R> a, Cx<R> 
 }
 // a * conj(b)
 template <class R> __device__ __forceinline__ Cx<R> mulconj(Cx<R> a, Cx<R> b) {
-  return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+  return {fma(a.re, b.re, a.im * b.im), fma(a.im, b.re, -(a.re * b.im))};
 }
 
 template <class P> struct PT { using R = P; static constexpr bool cplx = false; };
@@ -81,7 +85,7 @@ template <> __device__ __forceinline__ Cx<double> pzero<Cx<double>>() { return {
 
 __device__ __forceinline__ float abs2(float a) { return a * a; }
 __device__ __forceinline__ double abs2(double a) { return a * a; }
-template <class R> __device__ __forceinline__ R abs2(Cx<R> a) { return a.re * a.re + a.im * a.im; }
+template <class R> __device__ __forceinline__ R abs2(Cx<R> a) { return fma(a.re, a.re, a.im * a.im); }  // (as the products above)
 __device__ __forceinline__ double dabs2(float a) { return double(a) * double(a); }
 __device__ __forceinline__ double dabs2(double a) { return a * a; }
 template <class R> __device__ __forceinline__ double dabs2(Cx<R> a) {
@@ -1530,6 +1534,30 @@ __device__ __forceinline__ void copy_to_lds(T* dst, A* src, int n) {
 // The gap-evaluation variant holds extra FP64 accumulators; it is asked to
 // fit five waves per SIMD, as many as the plain variant, so that the same
 // tile shapes keep every block resident.
+// Dual step (TV or TGV, MODE) of a single dense slice done by workgroups of
+// their own, launched after the item workgroups of a pass: workgroup b of nd
+// takes a contiguous pixel range, and the ranges go to XCDs in consecutive
+// runs, so that the next row's pixels (forward differences) are mostly in
+// the same L2.  Gap terms (GAP) to part[b].
+template <class R, int MODE, bool GAP, int NT>
+__device__ __forceinline__ void dual_workgroup(const SliceDev& d, int nd, double* part) {
+  using C = Cx<R>;
+  const int b = xcd_item(int(blockIdx.x) - int(gridDim.x) + nd, nd), W = d.W, WH = W * d.H;
+  const int chunk = (WH + nd - 1) / nd;
+  const int pa = min(WH, b * chunk), pe = min(WH, pa + chunk);
+  const int k = gptr<int>(d.ctl)[CTL_ITER];
+  const R sigma = R(d.sigma);
+  const R ia0 = rcp(sched_alpha<R>(d, d.alpha0, d.alpha0_end, k));
+  const R ia1 = rcp(sched_alpha<R>(d, d.alpha1, d.alpha1_end, k));
+  const float rW = 1.0f / float(W);
+  double t[kGapTerms] = {0};
+  for (int i = pa + threadIdx.x; i < pe; i += NT) {
+    const int y = qdiv(i, W, rW), x = i - y * W;
+    dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, ia0, ia1, t);
+  }
+  if (GAP) block_sum<3, NT>(reinterpret_cast<double(&)[3]>(t), part + size_t(b) * kGapTerms);
+}
+
 template <class R, int MODE, bool GAP, int NT, int NF>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GAP ? 5 : 1))) void k_rows_fwd(const SliceDev* sl, const int2* items, int nitems, int cb,
                                                  double* part, int ndual) {
@@ -1538,22 +1566,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GAP ? 5 : 1)
   if (MODE != M_OP && int(blockIdx.x) >= nitems) {
     const SliceDev& d = sl[0];
     if (!slice_active(d)) return;
-    // consecutive pixel ranges on one XCD, so that the next row's pixels
-    // (forward differences) are mostly in the same L2
-    const int b = xcd_item(int(blockIdx.x) - nitems, ndual), W = d.W, WH = W * d.H;
-    const int chunk = (WH + ndual - 1) / ndual;
-    const int pa = min(WH, b * chunk), pe = min(WH, pa + chunk);
-    const int k = gptr<int>(d.ctl)[CTL_ITER];
-    const R sigma = R(d.sigma);
-    const R ia0 = rcp(sched_alpha<R>(d, d.alpha0, d.alpha0_end, k));
-    const R ia1 = rcp(sched_alpha<R>(d, d.alpha1, d.alpha1_end, k));
-    const float rW = 1.0f / float(W);
-    double t[kGapTerms] = {0};
-    for (int i = pa + threadIdx.x; i < pe; i += NT) {
-      const int y = qdiv(i, W, rW), x = i - y * W;
-      dual_pixel<C, C, MODE == M_TGV, GAP>(d, x, y, sigma, ia0, ia1, t);
-    }
-    if (GAP) block_sum<3, NT>(reinterpret_cast<double(&)[3]>(t), part + size_t(nitems + b) * kGapTerms);
+    dual_workgroup<R, MODE, GAP, NT>(d, ndual, part + size_t(nitems) * kGapTerms);
     return;
   }
   const int li = xcd_item(blockIdx.x, nitems);
@@ -1653,11 +1666,23 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GAP ? 5 : 1)
 // the prox loads as outstanding before the FFT.
 enum { CM_SOLVE = 0, CM_FWD = 1, CM_ADJ = 2 };
 
-template <class R, int CMODE, bool GAP, int NT, int NF, bool PF = false>
+// DUAL (M_TV or M_TGV; a single dense slice, no gap evaluation): ndual
+// workgroups after the column items do the dual step of the iteration
+// (dual_workgroup), which reads and writes none of what the column items do;
+// with the adjoint row pass also writing the next T1 (k_rows_adj NEXT) an
+// iteration is these two launches.
+template <class R, int CMODE, bool GAP, int NT, int NF, bool PF = false, int DUAL = 0>
 __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* items, int nitems, int cg,
-                                             double* part) {
+                                             double* part, int ndual) {
   static_assert(!PF || (NF > 0 && CMODE == CM_SOLVE), "register prefetch needs a compile-time length");
+  static_assert(DUAL == 0 || (CMODE == CM_SOLVE && !GAP), "dual workgroups: solver iterations without the gap");
   extern __shared__ char smem_raw[];
+  if (DUAL != 0 && int(blockIdx.x) >= nitems) {
+    const SliceDev& d = sl[0];
+    if (!slice_active(d)) return;
+    dual_workgroup<R, DUAL, false, NT>(d, ndual, nullptr);
+    return;
+  }
   const int li = xcd_item(blockIdx.x, nitems);
   const int2 it = items ? items[li] : make_int2(0, li);  // null: the dense single-slice list
   const SliceDev& d = sl[it.x];
@@ -1833,7 +1858,7 @@ constexpr int kRowsAdjFusedSlots = 5;
 // (TGV_primal_u_noprox_kernel, tgv_kernels.cuh:432-467).  Load order: the
 // twiddles, the T2 gather, u and p of the primal step (their arithmetic
 // waits for them), then the sensitivities, which the FFT does not need.
-template <class R, int MODE, bool GAP, int NT, int NF>
+template <class R, int MODE, bool GAP, int NT, int NF, bool NEXT>
 __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double* part_rows, char* smem_raw) {
   using C = Cx<R>;
   constexpr int W = NF, PS = kRowsAdjFusedSlots, PT = rows_adj_pt<NT>();
@@ -1918,14 +1943,44 @@ __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double*
     C w = other[x];
     for (int q = 1; q < ngs; ++q) w = w + other[q * W + x];
     const C un = e_uo + tau * (e_div - w);
+    const C ubn = (R(1) + theta) * un - theta * e_uo;
     gu[i] = un;
-    gub[i] = (R(1) + theta) * un - theta * e_uo;
+    gub[i] = ubn;
     if (GAP) t[0] += dabs2(w - e_div);
+    // the inverse transform is consumed (products above the barrier)
+    if (NEXT) res[x] = ubn;
   }
   if (GAP) block_sum<2, NT>(t, part_rows + size_t(row_beg + y) * kGapTerms);
   if (y == 0 && threadIdx.x == 0) {
     gctl[CTL_ITER] += 1;
     gctl[CTL_RUPD] += 1;
+  }
+  if constexpr (NEXT) {
+    // The forward row pass of the next iteration on the new ubar row (what
+    // k_rows_fwd's row items compute from it, the same operations in the
+    // same order): S_c * ubar -> ifftshift -> PE-FFT -> fftshift -> the
+    // sampled columns -> T1 [c][ks][y].  The sensitivities are still in
+    // registers, and the T1 entries are those of the T2 gather (e = c nc + ks).
+    __syncthreads();  // ubar row in res; the coil sums in other are read
+    if (gi < ngs) {
+      const C ubx = res[gx];
+#pragma unroll
+      for (int k = 0; k < PS; ++k) {
+        const int cl = gi * ch + k;
+        if (k < ch && cl < ncb) other[lp<NF, false>(cl * W + ishift_dst(gx, W))] = ubx * pS[k];
+      }
+    }
+    __syncthreads();
+    C* fres = fft_n<R, NT, false, NF, false>(other, res, ncb, d.planW, L.tw);
+    const auto T1 = gptr<C>(d.T1) + y;
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < ncb * nc) {
+        const int cl = qdiv(e, nc, rnc);
+        T1[size_t(e) * H] = fres[lp<NF, false>(cl * W + fshift_src(pK[k], W))];
+      }
+    }
   }
 }
 
@@ -1945,12 +2000,16 @@ __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double*
 // so the compiler's waits can leave the sensitivity loads, issued after the
 // T2 gather and the primal-step loads, in flight during the inverse FFT (the
 // other paths' runtime branches make it wait for every load before it).
-template <class R, int MODE, bool GAP, int NT, int NF, bool FUSED = false>
+// NEXT (with FUSED, no gap evaluation): the row blocks also compute the
+// next iteration's T1 from their new ubar row, which k_rows_fwd's row items
+// would otherwise do (rows_adj_fused).
+template <class R, int MODE, bool GAP, int NT, int NF, bool FUSED = false, bool NEXT = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1024 && NF > 0 && ct_pad_ok(NF) ? 8 : 1))) void k_rows_adj(const SliceDev* sl, const int2* items, int nitems, int cb,
                                                  double* part_rows, int nvstep) {
   extern __shared__ char smem_raw[];
   using C = Cx<R>;
   static_assert(!FUSED || (NF > 0 && MODE != M_OP && NT == kRowAdjThreadsWide), "fused adjoint row path");
+  static_assert(!NEXT || (FUSED && !GAP), "next forward rows: the fused path, solver iterations without the gap");
   if (int(blockIdx.x) >= nitems) {
     // nvstep workgroups after the row items (a single slice): the v step of
     // TGV_primal_v_kernel on contiguous pixel ranges, where the row items
@@ -1979,7 +2038,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1024 &
   const int2 it = items ? items[li] : make_int2(0, li);  // null: the dense single-slice list
   const SliceDev& d = sl[it.x];
   if constexpr (FUSED) {
-    rows_adj_fused<R, MODE, GAP, NT, NF>(d, it.y, part_rows, smem_raw);
+    rows_adj_fused<R, MODE, GAP, NT, NF, NEXT>(d, it.y, part_rows, smem_raw);
     return;
   }
   const int W = NF > 0 ? NF : d.W, H = d.H, nc = d.ncols;
@@ -2588,13 +2647,13 @@ static void mri_stage_m(const Launch& L, bool gap, int stage) {
       const int2* items = L.dense ? nullptr : L.cols;
       if constexpr (NF > 0) {
         if (L.cg <= kColPrefetchCols) {
-          if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF, true>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1]);
-          else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF, true>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1]);
+          if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF, true>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1], 0);
+          else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF, true>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1], 0);
           return;
         }
       }
-      if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1]);
-      else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1]);
+      if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1], 0);
+      else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1], 0);
     });
   } else if (stage == 2) {
     with_fft_len<R>(L.fft_w, [&](auto nf) {
@@ -2612,6 +2671,40 @@ template <class R>
 static void mri_stage(const Launch& L, int kind, bool gap, int stage) {
   if (kind == K_TV_OP) mri_stage_m<R, M_TV>(L, gap, stage);
   else mri_stage_m<R, M_TGV>(L, gap, stage);
+}
+
+// The two launches of a paired iteration (L.pair): 0 the column pass with
+// the dual step's workgroups, 1 the fused adjoint row pass that also writes
+// the next T1.
+template <class R, int MODE>
+static void mri_pair_m(const Launch& L, int which) {
+  constexpr int NC = kColThreads, NW = kRowAdjThreadsWide;
+  if (which == 0) {
+    const size_t lds = lds_occupancy(L.lds_cols, L.wpc[WPC_COLS], init_kernel_attrs());
+    with_fft_len<R>(L.fft_h, [&](auto nf) {
+      constexpr int NF = decltype(nf)::value;
+      if constexpr (NF > 0) {
+        if (L.cg <= kColPrefetchCols) {
+          PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF, true, MODE>), L.ncolitems + L.ndual, NC, lds, L.stream, L.sl, nullptr, L.ncolitems, L.cg, L.part[1], L.ndual);
+          return;
+        }
+      }
+      PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF, false, MODE>), L.ncolitems + L.ndual, NC, lds, L.stream, L.sl, nullptr, L.ncolitems, L.cg, L.part[1], L.ndual);
+    });
+  } else {
+    const size_t lds = lds_occupancy(L.lds_rows_a, L.wpc[WPC_ADJ], init_kernel_attrs());
+    with_fft_len<R>(L.fft_w, [&](auto nf) {
+      constexpr int NF = decltype(nf)::value;
+      if constexpr (NF > 0)
+        PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NW, NF, true, true>), L.nrg_a + L.nvstep, NW, lds, L.stream, L.sl, nullptr, L.nrg_a, L.cb_a, L.gap_rows, L.nvstep);
+    });
+  }
+}
+
+void run_pair_stage(const Launch& L, int kind, int dtype, int which) {
+  if (dtype != DT_C64) return;  // pairs need the compile-time row plans (single precision)
+  if (kind == K_TV_OP) mri_pair_m<float, M_TV>(L, which);
+  else mri_pair_m<float, M_TGV>(L, which);
 }
 
 void run_stage(const Launch& L, int kind, int dtype, bool fp16, bool gap, int stage) {
@@ -2656,7 +2749,7 @@ static void op_forward(const Launch& L) {
   });
   with_fft_len<R>(L.fft_h, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;
-    PDT_LAUNCH_NT((k_cols<R, CM_FWD, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+    PDT_LAUNCH_NT((k_cols<R, CM_FWD, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1], 0);
   });
 }
 
@@ -2665,7 +2758,7 @@ static void op_adjoint(const Launch& L) {
   constexpr int NC = kColThreads;
   with_fft_len<R>(L.fft_h, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;
-    PDT_LAUNCH_NT((k_cols<R, CM_ADJ, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1]);
+    PDT_LAUNCH_NT((k_cols<R, CM_ADJ, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1], 0);
   });
   with_fft_len<R>(L.fft_w, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;
@@ -2705,12 +2798,24 @@ static bool set_lds_attrs(int b) {
       set((const void*)&k_rows_adj<R, M_TV, true, kRowAdjThreadsWide, NF, true>);
       set((const void*)&k_rows_adj<R, M_TGV, false, kRowAdjThreadsWide, NF, true>);
       set((const void*)&k_rows_adj<R, M_TGV, true, kRowAdjThreadsWide, NF, true>);
+      if constexpr (std::is_same<R, float>::value) {
+        set((const void*)&k_rows_adj<R, M_TV, false, kRowAdjThreadsWide, NF, true, true>);
+        set((const void*)&k_rows_adj<R, M_TGV, false, kRowAdjThreadsWide, NF, true, true>);
+      }
     }
     set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF>);
     set((const void*)&k_cols<R, CM_SOLVE, true, NC, NF>);
+    if constexpr (std::is_same<R, float>::value) {
+      set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF, false, M_TV>);
+      set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF, false, M_TGV>);
+    }
     if constexpr (NF > 0) {
       set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF, true>);
       set((const void*)&k_cols<R, CM_SOLVE, true, NC, NF, true>);
+      if constexpr (std::is_same<R, float>::value) {
+        set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF, true, M_TV>);
+        set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF, true, M_TGV>);
+      }
     }
     set((const void*)&k_cols<R, CM_FWD, false, NC, NF>);
     set((const void*)&k_cols<R, CM_ADJ, false, NC, NF>);

@@ -52,6 +52,11 @@ struct Launch {
   // straight-line adjoint row path (rows_adj_fused_ok; no v step in the row
   // blocks)
   bool adj_fused = false;
+  // with adj_fused and ndual > 0 (single precision): an iteration without
+  // the gap evaluation may run as two launches (run_pair_stage) instead of
+  // three, the dual step in the column pass and the next forward row FFTs in
+  // the adjoint row pass
+  bool pair = false;
   // workgroup size of the denoising half-iterations (256, 512 or 1024): the
   // smallest that covers rpi rows of the widest slice in one pass, at most 1024
   int den_threads = 256;
@@ -86,6 +91,12 @@ void run_iteration(const Launch& L, int kind, int dtype, bool fp16, bool gap);
 int num_stages(int kind);
 const char* stage_name(int kind, int stage);
 void run_stage(const Launch& L, int kind, int dtype, bool fp16, bool gap, int stage);
+// One launch of a paired MRI iteration (L.pair, no gap evaluation): which 0
+// = the column pass with the dual step (reads the T1 of the current ubar),
+// 1 = the adjoint row pass with the primal step that also writes the T1 of
+// the new ubar.  The first iteration of a run of pairs starts from stages 0
+// and 1, and the last ends with stage 2 (which writes no T1).
+void run_pair_stage(const Launch& L, int kind, int dtype, int which);
 // tb_k ROF or TGV (kind) iterations of a single dense slice in one launch
 // (tiles held in LDS with a tb_k-pixel halo), reading the state (ROF: u,
 // ubar, p; TGV: u, v, p, q) and writing its second copy with ubar (and vbar)

@@ -54,6 +54,7 @@ void load_locked() {
         if (ss >> t[0] >> t[1] >> t[2]) {
           for (int i = 0; i < 3; ++i) v.tb[i] = t[i];
           if (!(ss >> v.dual_px)) v.dual_px = 0;
+          if (!(ss >> v.unpaired)) v.unpaired = 0;
         }
       }
       g_cache[k] = v;
@@ -68,7 +69,8 @@ void save_locked() {
   for (const auto& kv : g_cache)
     out << kv.first.str() << " : " << kv.second.cb << ' ' << kv.second.cg << ' ' << kv.second.rpi << ' '
         << kv.second.cb_a << ' ' << kv.second.wpc[0] << ' ' << kv.second.wpc[1] << ' ' << kv.second.wpc[2] << ' '
-        << kv.second.wpc[3] << ' ' << kv.second.tb[0] << ' ' << kv.second.tb[1] << ' ' << kv.second.tb[2] << ' ' << kv.second.dual_px << '\n';
+        << kv.second.wpc[3] << ' ' << kv.second.tb[0] << ' ' << kv.second.tb[1] << ' ' << kv.second.tb[2] << ' ' << kv.second.dual_px << ' '
+        << kv.second.unpaired << '\n';
 }
 }  // namespace
 

@@ -51,7 +51,7 @@ class PdtStats(ct.Structure):
 class PdtTiles(ct.Structure):
     _fields_ = [("cb", ct.c_int32), ("cg", ct.c_int32), ("rpi", ct.c_int32), ("cb_a", ct.c_int32),
                 ("wpc", ct.c_int32 * 4), ("tb_x", ct.c_int32), ("tb_y", ct.c_int32), ("tb_k", ct.c_int32),
-                ("dual_px", ct.c_int32)]
+                ("dual_px", ct.c_int32), ("unpaired", ct.c_int32)]
 
 
 class PdtKernelProf(ct.Structure):

@@ -251,7 +251,7 @@ class _Optimizer:
 
 def _tiles_dict(t) -> dict:
     return {"cb": t.cb, "cg": t.cg, "rpi": t.rpi, "cb_a": t.cb_a, "wg_per_cu": list(t.wpc),
-            "tv_block": [t.tb_x, t.tb_y, t.tb_k], "dual_px": t.dual_px}
+            "tv_block": [t.tb_x, t.tb_y, t.tb_k], "dual_px": t.dual_px, "unpaired": t.unpaired}
 
 
 def autotune(optimizers, reps: int = 20) -> dict:
@@ -269,7 +269,7 @@ def autotune(optimizers, reps: int = 20) -> dict:
 
 
 def set_tiles(optimizers, cb: int = 0, cg: int = 0, rpi: int = 0, cb_a: int = 0, wg_per_cu=None,
-              tv_block=None, dual_px: int = 0) -> None:
+              tv_block=None, dual_px: int = 0, unpaired: int = 0) -> None:
     """Stores a manual tile choice for the shape of these optimizers (one
     batch) in the tile cache the autotuner uses: coils per forward and
     adjoint row item (cb, cb_a), columns per column item (cg), rows per
@@ -278,12 +278,15 @@ def set_tiles(optimizers, cb: int = 0, cg: int = 0, rpi: int = 0, cb_a: int = 0,
     tv_block: (tile x, tile y, iterations per launch) of the temporally
     blocked ROF / TGV denoising launches, None or iterations 0 = off
     (the one-iteration kernels); dual_px: pixels per
-    thread of the TGV denoising dual step (1, 2; 0 = the default)."""
+    thread of the TGV denoising dual step (1, 2; 0 = the default);
+    unpaired: 1 = an MRI iteration always as the three passes (forward rows,
+    columns, adjoint rows), 0 = two launches where the adjoint row pass
+    holds every coil of a row (the default)."""
     for o in optimizers:
         o._push_params()
     arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
     t = C.PdtTiles(int(cb), int(cg), int(rpi), int(cb_a), (ct.c_int32 * 4)(*(wg_per_cu or (0, 0, 0, 0))),
-                   *[int(v) for v in (tv_block or (0, 0, 0))], int(dual_px))
+                   *[int(v) for v in (tv_block or (0, 0, 0))], int(dual_px), int(unpaired))
     C.check(C.lib().pdt_set_tiles(arr, len(optimizers), ct.byref(t)))
 
 

@@ -236,6 +236,56 @@ def test_single_slice_step_workgroups_equal_batch(solver, coils, n):
             assert np.array_equal(b.getV(), single.getV())
 
 
+@pytest.mark.parametrize("solver,coils,n,gap_every,max_iter,graph", [
+    ("tgvmri", 15, 320, 50, 120, 1),   # runs of 49 pairs, a gap iteration, a short last run
+    ("tgvmri", 15, 320, 3, 20, 1),     # runs of two iterations: a first and a last one only
+    ("tgvmri", 15, 320, 50, 60, 0),    # direct launches
+    ("tvop", 15, 320, 50, 60, 1),
+    ("tgvmri", 20, 256, 0, 74, 1),     # no gap evaluation: chunks of 50 and 25
+    ("tgvmri", 8, 320, 7, 30, 1),
+])
+def test_paired_iterations_equal_three_passes(solver, coils, n, gap_every, max_iter, graph):
+    """A single slice whose adjoint row blocks hold every coil runs an
+    iteration without the gap evaluation as two launches (the column pass
+    with the dual step's workgroups, then the adjoint row pass that also
+    computes the next iteration's forward row FFTs from its new ubar row);
+    the first iteration of each run of them starts with the forward row pass
+    and the last ends without the next FFTs.  Against the same solve forced
+    to three passes per iteration (unpaired): the same iterates bit for bit,
+    the same stop and gap, over run lengths 1, 2, 6, 49 and 50 iterations."""
+    pdt = _pdt()
+    from primaldualtoolbox import synthetic as syn
+    case = syn.mri_case(seed=37, n=n, coils=coils)
+    a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
+    cls = {"tgvmri": pdt.mri.TgvMriOptimizer_2c3c, "tvop": pdt.mri.TvOptimizerWithOp_2c3c}[solver]
+    prm = dict(gap_every=gap_every, use_graph=graph, gap_tol=0.0)
+    if solver == "tvop":
+        prm["Lambda"] = 0.5
+    a = None if solver == "tvop" else a1
+    _, paired = _mri_solver(pdt, case, cls=cls, a1=a, max_iter=max_iter, **prm)
+    _, three = _mri_solver(pdt, case, cls=cls, a1=a, max_iter=max_iter, **prm)
+    try:
+        pdt.mri.set_tiles([paired], cb=5, cg=1, cb_a=coils)
+        names = [k["name"] for k in paired.profileKernels(3)]
+        paired.solve()
+        pdt.mri.set_tiles([three], cb=5, cg=1, cb_a=coils, unpaired=1)
+        names3 = [k["name"] for k in three.profileKernels(3)]
+        three.solve()
+    finally:
+        pdt.mri.set_tiles([paired])
+    print(f"  launches: {names} / {names3}", flush=True)
+    assert names == ["k_cols_dual", "k_rows_adj_next"]
+    assert names3 == ["k_rows_fwd", "k_cols", "k_rows_adj"]
+    sp, s3 = paired.getStats(), three.getStats()
+    print(f"  paired {sp['iters']} it. gap {sp['rel_gap']:.12e}; three passes {s3['iters']} it. "
+          f"gap {s3['rel_gap']:.12e}", flush=True)
+    assert sp["iters"] == s3["iters"] == max_iter + 1
+    assert sp["rel_gap"] == s3["rel_gap"]
+    assert np.array_equal(paired.getResult(), three.getResult())
+    if solver == "tgvmri":
+        assert np.array_equal(paired.getV(), three.getV())
+
+
 def test_batch_ragged_stops_equal_single_solves():
     """Slices of one batch that stop at different iterations (iteration
     limits and gap tolerances) leave the work lists as they finish; every
