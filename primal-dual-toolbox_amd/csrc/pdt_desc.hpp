@@ -64,7 +64,7 @@ struct SliceDev {
   void* rk;           // compact dual r [c][ks][kfe]
   const void* S;      // coil sensitivities {W, Hk, C}; null for the sampling op
   const void* Mc;     // compact mask [ks][kfe] (real)
-  void* T1;           // row-pass output [c][ks][y] (column-major per coil)
+  void* T1;           // row-pass output [c][y][ks] (row-major per coil, as T2)
   void* T2;           // column-pass output [c][y][ks]
   void* wpart;        // per-coil-group partial adjoints [g][y][x]
   const void* op_in;  // operator mode input

@@ -9,7 +9,7 @@
 //    kernels per iteration.
 //      rows_fwd  : dual p (and q) update of image row y, then for every coil
 //                  S_c * u_bar row, ifftshift, PE-FFT in LDS, fftshift, keep
-//                  the sampled PE columns                  -> T1 [c][ks][y]
+//                  the sampled PE columns                  -> T1 [c][y][ks]
 //      cols      : per coil and group of sampled columns: FE-FFT of the
 //                  zero-padded (read-out oversampled) column, mask, data prox
 //                  of r, mask, inverse FE-FFT, crop to the image window
@@ -1526,7 +1526,7 @@ __device__ __forceinline__ void copy_to_lds(T* dst, A* src, int n) {
 // y.  The groups of row y also perform the dual step of row y (TV or TGV),
 // each on its own range of pixels.  Then for
 // each coil of the group: S_c * src row -> ifftshift -> PE-FFT -> fftshift
-// -> sampled columns -> T1 [c][ks][y].
+// -> sampled columns -> T1 [c][y][ks].
 // With ndual > 0 (a single slice, dense items) the dual step is left to
 // ndual further workgroups after the nitems row items, each on a contiguous
 // range of pixels (gap terms at part[nitems + b]), so that the FFT chains of
@@ -1647,12 +1647,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GAP ? 5 : 1)
   }
   __syncthreads();
   C* res = fft_n<R, NT, false, NF>(L.a, L.b, ncb, d.planW, L.tw);
-  // T1 is column-major per coil, [c][ks][y], so that the column pass reads
-  // its columns contiguously; here each row writes one element per column
-  const auto T1 = gptr<C>(d.T1) + size_t(c0) * nc * H + y;  // coil c0, column 0, row y
+  // T1 is row-major per coil, [c][y][ks], like T2: a row writes nc
+  // contiguous entries per coil (the column pass gathers with a stride)
+  const auto T1 = gptr<C>(d.T1) + (size_t(c0) * H + y) * nc;  // coil c0, row y
   for (int e = threadIdx.x; e < ncb * nc; e += NT) {
     const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
-    T1[size_t(cl * nc + ks) * H] = res[lp<NF>(cl * W + fshift_src(L.cols[ks], W))];
+    T1[size_t(cl) * H * nc + ks] = res[lp<NF>(cl * W + fshift_src(L.cols[ks], W))];
   }
   if (GAP) block_sum<3, NT>(reinterpret_cast<double(&)[3]>(t), part + size_t(li) * kGapTerms);
 }
@@ -1716,8 +1716,8 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
   R pf_m[KP];
   constexpr bool pf = PF;
   // the gather of the zero-padded, ifftshifted columns of T1 (coil c, columns
-  // ks0.., contiguous in y): into registers with a compile-time length
-  const auto T1 = gptr<const C>(d.T1) + (size_t(c) * nc + ks0) * H;
+  // ks0.., a stride of nc in y): into registers with a compile-time length
+  const auto T1 = gptr<const C>(d.T1) + size_t(c) * H * nc + ks0;
   C pf_t[KP];
   if (pf) {
 #pragma unroll
@@ -1726,7 +1726,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
       if (e < ncg * Hk) {
         const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
         const int yy = ishift_src(j, Hk) - y0;
-        pf_t[k] = (yy >= 0 && yy < H) ? C(T1[cl * H + yy]) : pzero<C>();
+        pf_t[k] = (yy >= 0 && yy < H) ? C(T1[yy * nc + cl]) : pzero<C>();
       }
     }
     // branch-free (lanes past the columns load the last element, unused), so
@@ -1757,7 +1757,7 @@ __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* ite
         const int j = qdiv(e, ncg, rncg), cl = e - j * ncg;
         const int fe = ishift_src(j, Hk);
         const int yy = fe - y0;
-        bufA[lp<NF>(cl * Hk + j)] = (yy >= 0 && yy < H) ? C(T1[cl * H + yy]) : pzero<C>();
+        bufA[lp<NF>(cl * Hk + j)] = (yy >= 0 && yy < H) ? C(T1[yy * nc + cl]) : pzero<C>();
       }
     }
     if (NF > 0) s_tw.store(tw, Hk);
@@ -1886,15 +1886,20 @@ __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double*
   // u(x, y) and the neighbours of div p (p is final for this iteration),
   // loaded branch-free (threads past the row load pixel 0, unused)
   const R tau = R(d.tau), theta = R(d.theta);
-  const int x = threadIdx.x, xr = x < W ? x : 0;
+  // (NEXT: every thread group takes the u step of its own pixel gx, each
+  // forming the same values, so that its forward products need no exchange
+  // of the new ubar through LDS)
+  const int ngs = min(NT / W, ncb), ch = (ncb + ngs - 1) / ngs;
+  const int gi = threadIdx.x / W, gx = threadIdx.x - gi * W;
+  const int x = threadIdx.x;
+  const bool upix = NEXT ? gi < ngs : x < W;
+  const int xr = NEXT ? (gi < ngs ? gx : 0) : (x < W ? x : 0);
   const size_t n = size_t(W) * H, i = size_t(y) * W + xr;
   const auto pp = gptr<const C>(d.p);
   const C e_uo = gptr<const C>(d.u)[i];
   const C cx = pp[i], mx = pp[xr > 0 ? i - 1 : i];
   const C cy = pp[n + i], my = pp[n + (y > 0 ? i - W : i)];
   // sensitivities, branch-free (spare slots load a valid element, unused)
-  const int ngs = min(NT / W, ncb), ch = (ncb + ngs - 1) / ngs;
-  const int gi = threadIdx.x / W, gx = threadIdx.x - gi * W;
   const int G = W * d.Hk;
   const auto Sg = gptr<const C>(d.S) + size_t(y + d.y0) * W + gx;
   C pS[PS];
@@ -1939,16 +1944,26 @@ __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double*
   }
   __syncthreads();
   double t[2] = {0, 0};
-  if (x < W) {
-    C w = other[x];
-    for (int q = 1; q < ngs; ++q) w = w + other[q * W + x];
+  if (upix) {
+    C w = other[xr];
+    for (int q = 1; q < ngs; ++q) w = w + other[q * W + xr];
     const C un = e_uo + tau * (e_div - w);
     const C ubn = (R(1) + theta) * un - theta * e_uo;
-    gu[i] = un;
-    gub[i] = ubn;
+    if (!NEXT || gi == 0) {
+      gu[i] = un;
+      gub[i] = ubn;
+    }
     if (GAP) t[0] += dabs2(w - e_div);
-    // the inverse transform is consumed (products above the barrier)
-    if (NEXT) res[x] = ubn;
+    if constexpr (NEXT) {
+      // the next forward row pass's input (what k_rows_fwd's row items form
+      // from ubar, the same operations): S_c ubar, ifftshifted, into the
+      // inverse transform's buffer (consumed above the barrier)
+#pragma unroll
+      for (int k = 0; k < PS; ++k) {
+        const int cl = gi * ch + k;
+        if (k < ch && cl < ncb) res[lp<NF, false>(cl * W + ishift_dst(gx, W))] = ubn * pS[k];
+      }
+    }
   }
   if (GAP) block_sum<2, NT>(t, part_rows + size_t(row_beg + y) * kGapTerms);
   if (y == 0 && threadIdx.x == 0) {
@@ -1956,29 +1971,18 @@ __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double*
     gctl[CTL_RUPD] += 1;
   }
   if constexpr (NEXT) {
-    // The forward row pass of the next iteration on the new ubar row (what
-    // k_rows_fwd's row items compute from it, the same operations in the
-    // same order): S_c * ubar -> ifftshift -> PE-FFT -> fftshift -> the
-    // sampled columns -> T1 [c][ks][y].  The sensitivities are still in
-    // registers, and the T1 entries are those of the T2 gather (e = c nc + ks).
-    __syncthreads();  // ubar row in res; the coil sums in other are read
-    if (gi < ngs) {
-      const C ubx = res[gx];
-#pragma unroll
-      for (int k = 0; k < PS; ++k) {
-        const int cl = gi * ch + k;
-        if (k < ch && cl < ncb) other[lp<NF, false>(cl * W + ishift_dst(gx, W))] = ubx * pS[k];
-      }
-    }
-    __syncthreads();
-    C* fres = fft_n<R, NT, false, NF, false>(other, res, ncb, d.planW, L.tw);
-    const auto T1 = gptr<C>(d.T1) + y;
+    // The rest of the next iteration's forward row pass on the new ubar row:
+    // PE-FFT -> fftshift -> the sampled columns -> T1 [c][y][ks] (the
+    // entries of the T2 gather, e = c nc + ks, in the same layout).
+    __syncthreads();  // products in res; the coil sums in other are read
+    C* fres = fft_n<R, NT, false, NF, false>(res, other, ncb, d.planW, L.tw);
+    const auto T1 = gptr<C>(d.T1) + size_t(y) * nc;
 #pragma unroll
     for (int k = 0; k < PT; ++k) {
       const int e = threadIdx.x + k * NT;
       if (e < ncb * nc) {
         const int cl = qdiv(e, nc, rnc);
-        T1[size_t(e) * H] = fres[lp<NF, false>(cl * W + fshift_src(pK[k], W))];
+        T1[size_t(cl) * H * nc + (e - cl * nc)] = fres[lp<NF, false>(cl * W + fshift_src(pK[k], W))];
       }
     }
   }
