@@ -668,6 +668,51 @@ __device__ __forceinline__ Cx<R>* fft_n(Cx<R>* a, Cx<R>* b, int nseq, const FftP
   else return lds_fft<R, NT>(a, b, nseq, pl, tw, INV);
 }
 
+// One sequence of the compile-time plan transformed by one wavefront alone,
+// in place on its padded slots (lengths with ct_pad_ok, at most 64 M
+// butterflies per stage): each stage reads every input of the wave's
+// butterflies into registers before it writes an output, and a wave's LDS
+// accesses take effect in issue order, so no workgroup barrier separates the
+// stages.  The butterflies are fft_ct_stage's (butterfly_ct's operations),
+// so the result is that of fft_n on padded buffers.  seq: the sequence's
+// first slot; call with S = 0, P = 1.
+template <class R, int N, int S, int P, bool INV>
+__device__ __forceinline__ void fft_wave(Cx<R>* seq, const Cx<R>* tw) {
+  if constexpr (S < ct_nstages(N)) {
+    constexpr int r = ct_radix(N, S), nr = N / r, tws = N / (P * r);
+    static_assert(ct_pad_ok(N) && r <= 10, "wave transforms: padded lengths, register codelets");
+    constexpr int nrp = nr + nr / 8, pp = P > 1 ? P + P / 8 : P;
+    constexpr int M = (nr + 63) / 64;  // butterflies per lane
+    const int lane = threadIdx.x & 63;
+    Cx<R> x[M][r];
+    int ob[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int i = lane + 64 * m;
+      ob[m] = -1;
+      if (i < nr) {
+        const int k = i % P;
+        const int o = (i - k) * r + k;
+        const Cx<R>* in = seq + i + (i >> 3);
+        x[m][0] = in[0];
+#pragma unroll
+        for (int j = 1; j < r; ++j) x[m][j] = P > 1 ? in[j * nrp] * twd(tw, j * (k * tws), INV) : in[j * nrp];
+        Dft<R, r>::run(x[m], INV ? R(1) : R(-1));
+        ob[m] = o + (o >> 3);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+      if (ob[m] >= 0) {
+#pragma unroll
+        for (int q = 0; q < r; ++q) seq[ob[m] + q * pp] = x[m][q];
+      }
+    __builtin_amdgcn_wave_barrier();
+    fft_wave<R, N, S + 1, P * r, INV>(seq, tw);
+  }
+}
+
 // numpy shift maps: ifftshift moves i to (i + ceil(n/2)) % n (the prefft /
 // preifft index map of mricartesianoperator.cu:44-49); fftshift output k
 // reads input (k - n/2) mod n.
@@ -1862,25 +1907,49 @@ template <class R, int MODE, bool GAP, int NT, int NF, bool NEXT>
 __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double* part_rows, char* smem_raw) {
   using C = Cx<R>;
   constexpr int W = NF, PS = kRowsAdjFusedSlots, PT = rows_adj_pt<NT>();
+  // Lengths with the padded layout: one padded slot range per coil, whose
+  // T2 scatter, transforms (fft_wave) and T1 entries one wavefront does
+  // alone (coils wave, wave + NW, ...; lb holds the coil-group sums), so the
+  // block waits at four barriers.  Other lengths: two unpadded buffers and
+  // workgroup-wide stages (lp without padding is the identity for them).
+  constexpr bool IP = ct_pad_ok(NF);
+  constexpr int NW = NT / 64, NPW = W + W / 8;  // waves; padded slots per coil
+  constexpr int SPW = (kRowsAdjFusedSlots * (NT / NF) + NW - 1) / NW;  // coils per wave
   const int H = d.H, nc = d.ncols, ncb = d.C;
-  RowLds<C> L(smem_raw, W, ncb, false);
+  const int ngs = min(NT / W, ncb), ch = (ncb + ngs - 1) / ngs;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  C* const tw = reinterpret_cast<C*>(smem_raw);
+  C* const la = tw + W;
+  C* const lb = la + (IP ? ncb * NPW : ncb * W);
   Staged<C, NF, NT> s_tw;
   s_tw.load(gptr<const C>(d.planW.tw), W);
-  for (int e = threadIdx.x; e < ncb * W; e += NT) L.a[e] = pzero<C>();
-  // T2 row y of every coil and the PE column of each sampled entry
+  if constexpr (!IP)
+    for (int e = threadIdx.x; e < ncb * W; e += NT) la[e] = pzero<C>();
+  // T2 row y of every coil and the PE column of each sampled entry (padded
+  // layout: entries ks = lane + 64 k of the wave's coils, branch-free with
+  // clamped indices; else entry e = tid + k NT of the row)
   const float rnc = 1.0f / float(max(nc, 1));
   const auto T2 = gptr<const C>(d.T2) + size_t(y) * nc;
   const auto gcols = gptr<const int>(d.cols);
   const int cstride = H * nc;
-  C pT[PT];
+  C pT[IP ? SPW : 1][PT];
   int pK[PT];
+  if constexpr (IP) {
 #pragma unroll
-  for (int k = 0; k < PT; ++k) {
-    const int e = threadIdx.x + k * NT;
-    if (e < ncb * nc) {
-      const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
-      pT[k] = T2[cl * cstride + ks];
-      pK[k] = gcols[ks];
+    for (int k = 0; k < PT; ++k) pK[k] = gcols[min(lane + 64 * k, nc - 1)];
+#pragma unroll
+    for (int sq = 0; sq < SPW; ++sq)
+#pragma unroll
+      for (int k = 0; k < PT; ++k) pT[sq][k] = T2[min(wave + NW * sq, ncb - 1) * cstride + min(lane + 64 * k, nc - 1)];
+  } else {
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < ncb * nc) {
+        const int cl = qdiv(e, nc, rnc), ks = e - cl * nc;
+        pT[0][k] = T2[cl * cstride + ks];
+        pK[k] = gcols[ks];
+      }
     }
   }
   // u(x, y) and the neighbours of div p (p is final for this iteration),
@@ -1889,7 +1958,6 @@ __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double*
   // (NEXT: every thread group takes the u step of its own pixel gx, each
   // forming the same values, so that its forward products need no exchange
   // of the new ubar through LDS)
-  const int ngs = min(NT / W, ncb), ch = (ncb + ngs - 1) / ngs;
   const int gi = threadIdx.x / W, gx = threadIdx.x - gi * W;
   const int x = threadIdx.x;
   const bool upix = NEXT ? gi < ngs : x < W;
@@ -1920,25 +1988,57 @@ __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double*
   const C dy = csel(y > 0, csel(y < H - 1, cy - my, -my), cy);
   C e_div = dx + dy;
   asm volatile("" : "+v"(e_div.re), "+v"(e_div.im));
-  s_tw.store(L.tw, W);
-  __syncthreads();  // zero fill, twiddles
+  C* res;
+  C* other;
+  if constexpr (IP) {
+    // each wave: its coils' slots zeroed, their T2 entries at the
+    // ifftshifted PE slots, then (twiddles in) their inverse transforms
 #pragma unroll
-  for (int k = 0; k < PT; ++k) {
-    const int e = threadIdx.x + k * NT;
-    if (e < ncb * nc) {
-      const int cl = qdiv(e, nc, rnc);
-      L.a[lp<NF, false>(cl * W + ishift_dst(pK[k], W))] = pT[k];
+    for (int sq = 0; sq < SPW; ++sq) {
+      const int cl = wave + NW * sq;
+      if (cl < ncb) {
+        C* const sp = la + cl * NPW;
+        for (int e = lane; e < NPW; e += 64) sp[e] = pzero<C>();
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int k = 0; k < PT; ++k) {
+          const int j = ishift_dst(pK[k], W);
+          if (lane + 64 * k < nc) sp[j + (j >> 3)] = pT[sq][k];
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
     }
+    s_tw.store(tw, W);
+    __syncthreads();  // twiddles
+#pragma unroll
+    for (int sq = 0; sq < SPW; ++sq) {
+      const int cl = wave + NW * sq;
+      if (cl < ncb) fft_wave<R, NF, 0, 1, true>(la + cl * NPW, tw);
+    }
+    res = la;
+    other = lb;
+  } else {
+    s_tw.store(tw, W);
+    __syncthreads();  // zero fill, twiddles
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < ncb * nc) {
+        const int cl = qdiv(e, nc, rnc);
+        la[lp<NF, false>(cl * W + ishift_dst(pK[k], W))] = pT[0][k];
+      }
+    }
+    __syncthreads();
+    res = fft_n<R, NT, true, NF, false>(la, lb, ncb, d.planW, tw);
+    other = (res == la) ? lb : la;
   }
-  __syncthreads();
-  C* res = fft_n<R, NT, true, NF, false>(L.a, L.b, ncb, d.planW, L.tw);
-  C* other = (res == L.a) ? L.b : L.a;
+  __syncthreads();  // every coil's inverse transform (fft_n: its last stage's)
   if (gi < ngs) {
     C acc = pzero<C>();
 #pragma unroll
     for (int k = 0; k < PS; ++k) {
       const int cl = gi * ch + k;
-      if (k < ch && cl < ncb) acc = acc + mulconj(res[lp<NF, false>(cl * W + fshift_src(gx, W))], pS[k]);
+      if (k < ch && cl < ncb) acc = acc + mulconj(res[lp<NF, IP>(cl * W + fshift_src(gx, W))], pS[k]);
     }
     other[gi * W + gx] = acc;
   }
@@ -1961,7 +2061,7 @@ __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double*
 #pragma unroll
       for (int k = 0; k < PS; ++k) {
         const int cl = gi * ch + k;
-        if (k < ch && cl < ncb) res[lp<NF, false>(cl * W + ishift_dst(gx, W))] = ubn * pS[k];
+        if (k < ch && cl < ncb) res[lp<NF, IP>(cl * W + ishift_dst(gx, W))] = ubn * pS[k];
       }
     }
   }
@@ -1973,16 +2073,32 @@ __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double*
   if constexpr (NEXT) {
     // The rest of the next iteration's forward row pass on the new ubar row:
     // PE-FFT -> fftshift -> the sampled columns -> T1 [c][y][ks] (the
-    // entries of the T2 gather, e = c nc + ks, in the same layout).
+    // entries of the T2 gather, in the same layout).
     __syncthreads();  // products in res; the coil sums in other are read
-    C* fres = fft_n<R, NT, false, NF, false>(res, other, ncb, d.planW, L.tw);
     const auto T1 = gptr<C>(d.T1) + size_t(y) * nc;
+    if constexpr (IP) {
 #pragma unroll
-    for (int k = 0; k < PT; ++k) {
-      const int e = threadIdx.x + k * NT;
-      if (e < ncb * nc) {
-        const int cl = qdiv(e, nc, rnc);
-        T1[size_t(cl) * H * nc + (e - cl * nc)] = fres[lp<NF, false>(cl * W + fshift_src(pK[k], W))];
+      for (int sq = 0; sq < SPW; ++sq) {
+        const int cl = wave + NW * sq;
+        if (cl < ncb) {
+          C* const sp = res + cl * NPW;
+          fft_wave<R, NF, 0, 1, false>(sp, tw);
+#pragma unroll
+          for (int k = 0; k < PT; ++k) {
+            const int ks = lane + 64 * k, j = fshift_src(pK[k], W);
+            if (ks < nc) T1[size_t(cl) * H * nc + ks] = sp[j + (j >> 3)];
+          }
+        }
+      }
+    } else {
+      C* fres = fft_n<R, NT, false, NF, false>(res, other, ncb, d.planW, tw);
+#pragma unroll
+      for (int k = 0; k < PT; ++k) {
+        const int e = threadIdx.x + k * NT;
+        if (e < ncb * nc) {
+          const int cl = qdiv(e, nc, rnc);
+          T1[size_t(cl) * H * nc + (e - cl * nc)] = fres[lp<NF, false>(cl * W + fshift_src(pK[k], W))];
+        }
       }
     }
   }
@@ -2604,7 +2720,9 @@ bool rows_adj_fused_ok(int dtype, int W, int C, int nc) {
   constexpr int NW = kRowAdjThreadsWide;
   if (!fft_len_compiled(dtype, W) || 2 * W > NW || C < 1) return false;
   const int ngs = std::min(NW / W, C), ch = (C + ngs - 1) / ngs;
-  return ch <= kRowsAdjFusedSlots && C * nc <= rows_adj_pt<NW>() * NW;
+  // (padded lengths: a wave holds the listed columns of its coils, 64 per slot)
+  const bool cols_fit = ct_pad_ok(W) ? nc <= 64 * rows_adj_pt<NW>() : C * nc <= rows_adj_pt<NW>() * NW;
+  return ch <= kRowsAdjFusedSlots && cols_fit;
 }
 
 // Adjoint row pass with L.adj_threads threads per block (kRowThreads, or
