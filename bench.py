@@ -653,6 +653,8 @@ def main():
     # per-kernel durations (hipEvents on the solver stream) and algorithmic bytes
     prof = opt.profileKernels(args.profile_reps)
     per_iter_ms = sum(p["ms"] for p in prof)
+    iter_bytes = sum(p["bytes"] for p in prof)
+    iter_gbs = iter_bytes / (per_iter_ms * 1e-3) / 1e9
     dom = max(prof, key=lambda p: p["ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     traffic = None
@@ -718,6 +720,11 @@ def main():
             "traffic_source": ("static: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel, committed in "
                                "profiles/traffic_tgv_mri_320x320x15.json (not measured in this run)"
                                if traffic is not None else None),
+            # every launch of an iteration together: algorithmic bytes of all
+            # of them over their summed durations (a paired iteration moves
+            # fewer bytes in two launches than three passes do)
+            "iteration": {"launches": [p["name"] for p in prof], "bytes": iter_bytes,
+                          "us": per_iter_ms * 1e3, "achieved": iter_gbs, "frac": iter_gbs / HBM_PEAK_GBS},
         },
         "cpu_baseline": cpu,
         "extra": extra,
