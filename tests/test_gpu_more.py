@@ -236,15 +236,18 @@ def test_single_slice_step_workgroups_equal_batch(solver, coils, n):
             assert np.array_equal(b.getV(), single.getV())
 
 
-@pytest.mark.parametrize("solver,coils,n,gap_every,max_iter,graph", [
-    ("tgvmri", 15, 320, 50, 120, 1),   # runs of 49 pairs, a gap iteration, a short last run
-    ("tgvmri", 15, 320, 3, 20, 1),     # runs of two iterations: a first and a last one only
-    ("tgvmri", 15, 320, 50, 60, 0),    # direct launches
-    ("tvop", 15, 320, 50, 60, 1),
-    ("tgvmri", 20, 256, 0, 74, 1),     # no gap evaluation: chunks of 50 and 25
-    ("tgvmri", 8, 320, 7, 30, 1),
+@pytest.mark.parametrize("solver,coils,n,n_pe,gap_every,max_iter,graph", [
+    ("tgvmri", 15, 320, 320, 50, 120, 1),  # runs of 49 pairs, a gap iteration, a short last run
+    ("tgvmri", 15, 320, 320, 3, 20, 1),    # runs of two iterations: a first and a last one only
+    ("tgvmri", 15, 320, 320, 50, 60, 0),   # direct launches
+    ("tvop", 15, 320, 320, 50, 60, 1),
+    ("tgvmri", 20, 256, 256, 0, 74, 1),    # no gap evaluation: chunks of 50 and 25; two coils per wave
+    ("tgvmri", 8, 320, 320, 7, 30, 1),
+    ("tgvmri", 6, 384, 384, 50, 30, 1),    # rows of 384 (radix-6 last stage)
+    ("tgvmri", 4, 64, 512, 50, 30, 1),     # rows of 512
+    ("tgvmri", 10, 368, 368, 50, 30, 1),   # rows of 368: two unpadded buffers, workgroup-wide stages
 ])
-def test_paired_iterations_equal_three_passes(solver, coils, n, gap_every, max_iter, graph):
+def test_paired_iterations_equal_three_passes(solver, coils, n, n_pe, gap_every, max_iter, graph):
     """A single slice whose adjoint row blocks hold every coil runs an
     iteration without the gap evaluation as two launches (the column pass
     with the dual step's workgroups, then the adjoint row pass that also
@@ -252,10 +255,12 @@ def test_paired_iterations_equal_three_passes(solver, coils, n, gap_every, max_i
     the first iteration of each run of them starts with the forward row pass
     and the last ends without the next FFTs.  Against the same solve forced
     to three passes per iteration (unpaired): the same iterates bit for bit,
-    the same stop and gap, over run lengths 1, 2, 6, 49 and 50 iterations."""
+    the same stop and gap, over run lengths 1, 2, 6, 49 and 50 iterations and
+    every row length of the all-coil adjoint path (256, 320, 384 and 512 with
+    per-wave transforms on padded slots, 368 with workgroup-wide stages)."""
     pdt = _pdt()
     from primaldualtoolbox import synthetic as syn
-    case = syn.mri_case(seed=37, n=n, coils=coils)
+    case = syn.mri_case(seed=37, n=n, n_pe=n_pe, coils=coils)
     a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
     cls = {"tgvmri": pdt.mri.TgvMriOptimizer_2c3c, "tvop": pdt.mri.TvOptimizerWithOp_2c3c}[solver]
     prm = dict(gap_every=gap_every, use_graph=graph, gap_tol=0.0)
