@@ -196,8 +196,14 @@ PDT_API int pdt_solver_destroy(pdt_solver* s);
  * bytes of one launch (every distinct element the launch reads or writes,
  * once, at storage precision).  Stages that launch nothing for this solver
  * (the separate primal kernel when every slice has one coil group) are
- * omitted; *n receives the number of entries written.  Runs from the
- * solver's current state and puts its iterates back at the end. */
+ * omitted; *n receives the number of entries written.  A single MRI slice
+ * that runs paired iterations (pdt_tiles.unpaired) reports their two
+ * launches, "k_cols_dual" and "k_rows_adj_next"; otherwise the stages
+ * ("k_rows_fwd", "k_cols", "k_rows_adj", "k_mri_primal"; denoising:
+ * "k_tv_dual"/"k_tv_primal", "k_tgv_primal"/"k_tgv_dual", or one
+ * "k_tv_block"/"k_tgv_block" entry per launch of the temporally blocked
+ * path).  Runs from the solver's current state and puts its iterates back
+ * at the end. */
 typedef struct pdt_kernel_prof {
   char name[32];
   double ms;

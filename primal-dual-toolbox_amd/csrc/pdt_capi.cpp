@@ -834,6 +834,7 @@ struct Engine {
   struct GraphEntry {
     int len;
     bool gap;
+    bool first;  // paired iterations: the chunk starts a solve (forward row pass first)
     hipGraphExec_t exec;
   };
   std::vector<GraphEntry> graphs;
@@ -1035,6 +1036,9 @@ struct Engine {
     const size_t row_items = rows.size() + (mri && ns == 1 ? size_t(vstep_groups(hs[0])) : 0);
     const size_t nitem = std::max<size_t>(1, std::max({row_items, rg_items, rgs_a.size(), cols.size()}));
     d_part.alloc(4 * nitem * kGapTerms * sizeof(double));
+    // (a paired gap iteration leaves the forward row items' partial sums,
+    // zero in a single-slice solve, unwritten: they start at zero)
+    HIPC(hipMemset(d_part.p, 0, 4 * nitem * kGapTerms * sizeof(double)));
     L.stream = stream;
     L.sl = d_slices.as<SliceDev>();
     L.nsl = ns;
@@ -1291,7 +1295,7 @@ struct Engine {
     if (paired()) stage(0, false);
   }
   void iter_launch(int st) {
-    if (paired()) run_pair_stage(G[0], kind, dtk(), st);
+    if (paired()) run_pair_stage(G[0], kind, dtk(), st, false);
     else stage(st, false);
   }
   std::string iter_launch_name(int st) const {
@@ -1310,10 +1314,13 @@ struct Engine {
   }
 
   // Enqueues `len` iterations; the last one evaluates the gap when asked.
+  // first: the iterations start a solve (paired iterations take the T1 of
+  // the first one from the forward row pass, later chunks from the last
+  // adjoint row pass of the chunk before).
   // With temporally blocked ROF launches, the iterations before the gap
   // iteration run as pairs of launches of tb_k iterations each (the state
   // goes to the second copy and back), the rest one iteration at a time.
-  void enqueue(int len, bool gap) {
+  void enqueue(int len, bool gap, bool first = true) {
     const int dt = dtk();
     if (blocked()) {
       const int k = G[0].tb_k;
@@ -1330,26 +1337,20 @@ struct Engine {
       return;
     }
     if (paired()) {
-      // iterations before the gap iteration in pairs of launches: the first
-      // takes its T1 from the forward row pass (which also does its dual
-      // step), every other from the adjoint row pass before it; the last
-      // one's adjoint row pass writes no T1, as the next iteration (the gap
-      // iteration, or the first of the next chunk) starts with the forward
-      // row pass again
-      const int plain = len - (gap ? 1 : 0);
-      for (int i = 0; i < plain; ++i) {
-        if (i == 0) {
-          stage(0, false);
-          stage(1, false);
+      // every iteration in a pair of launches, the gap iteration included;
+      // each adjoint row pass writes the T1 of the next iteration, so only
+      // the first iteration of a solve takes its T1 from the forward row pass
+      // (which also does its dual step)
+      for (int i = 0; i < len; ++i) {
+        const bool g = gap && i == len - 1;
+        if (i == 0 && first) {
+          stage(0, g);
+          stage(1, g);
         } else {
-          run_pair_stage(G[0], kind, dt, 0);
+          run_pair_stage(G[0], kind, dt, 0, g);
         }
-        if (i == plain - 1) stage(2, false);
-        else run_pair_stage(G[0], kind, dt, 1);
-      }
-      if (gap) {
-        iteration(true);
-        run_gap_reduce(L, kind, dt);
+        run_pair_stage(G[0], kind, dt, 1, g);
+        if (g) run_gap_reduce(L, kind, dt);
       }
       return;
     }
@@ -1364,25 +1365,26 @@ struct Engine {
   // out like d_cs, in one copy (stream-ordered after the chunk enqueued before).
   void copy_ctl(char* host) { HIPC(hipMemcpyAsync(host, d_cs.p, cs_bytes, hipMemcpyDeviceToHost, stream)); }
 
-  void launch_chunk(int len, bool gap, bool use_graph) {
+  void launch_chunk(int len, bool gap, bool use_graph, bool first) {
+    first = first && paired();  // only paired iterations start differently
     if (!use_graph) {
-      enqueue(len, gap);
+      enqueue(len, gap, first);
       HIPC(hipGetLastError());
       return;
     }
     for (auto& g : graphs)
-      if (g.len == len && g.gap == gap) {
+      if (g.len == len && g.gap == gap && g.first == first) {
         HIPC(hipGraphLaunch(g.exec, stream));
         return;
       }
     hipGraph_t graph;
     HIPC(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    enqueue(len, gap);
+    enqueue(len, gap, first);
     HIPC(hipStreamEndCapture(stream, &graph));
     hipGraphExec_t exec;
     HIPC(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     (void)hipGraphDestroy(graph);
-    graphs.push_back({len, gap, exec});
+    graphs.push_back({len, gap, first, exec});
     HIPC(hipGraphLaunch(exec, stream));
   }
 
@@ -1454,7 +1456,7 @@ struct Engine {
       while (true) {
         while (live && k < total && nq < (converging() ? 1 : 2)) {
           const unsigned b = boundary(k);
-          launch_chunk(int(b - k), gap && (b % gap_every) == 0, use_graph);
+          launch_chunk(int(b - k), gap && (b % gap_every) == 0, use_graph, k == 0);
           const int q = next;
           next ^= 1;
           copy_ctl(h_cs_q[q]);
@@ -1482,7 +1484,7 @@ struct Engine {
       const auto t0 = std::chrono::steady_clock::now();
       while (k < total) {
         const unsigned b = boundary(k);
-        launch_chunk(int(b - k), gap && (b % gap_every) == 0, use_graph);
+        launch_chunk(int(b - k), gap && (b % gap_every) == 0, use_graph, k == 0);
         copy_ctl(h_cs);
         HIPC(hipStreamSynchronize(stream));
         k = b;

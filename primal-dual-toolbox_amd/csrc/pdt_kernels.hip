@@ -1711,21 +1711,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GAP ? 5 : 1)
 // the prox loads as outstanding before the FFT.
 enum { CM_SOLVE = 0, CM_FWD = 1, CM_ADJ = 2 };
 
-// DUAL (M_TV or M_TGV; a single dense slice, no gap evaluation): ndual
-// workgroups after the column items do the dual step of the iteration
-// (dual_workgroup), which reads and writes none of what the column items do;
+// DUAL (M_TV or M_TGV; a single dense slice): ndual workgroups after the
+// column items do the dual step of the iteration (dual_workgroup, gap terms
+// to part_dual), which reads and writes none of what the column items do;
 // with the adjoint row pass also writing the next T1 (k_rows_adj NEXT) an
 // iteration is these two launches.
 template <class R, int CMODE, bool GAP, int NT, int NF, bool PF = false, int DUAL = 0>
 __global__ __launch_bounds__(NT) void k_cols(const SliceDev* sl, const int2* items, int nitems, int cg,
-                                             double* part, int ndual) {
+                                             double* part, int ndual, double* part_dual) {
   static_assert(!PF || (NF > 0 && CMODE == CM_SOLVE), "register prefetch needs a compile-time length");
-  static_assert(DUAL == 0 || (CMODE == CM_SOLVE && !GAP), "dual workgroups: solver iterations without the gap");
+  static_assert(DUAL == 0 || CMODE == CM_SOLVE, "dual workgroups: solver iterations");
   extern __shared__ char smem_raw[];
   if (DUAL != 0 && int(blockIdx.x) >= nitems) {
     const SliceDev& d = sl[0];
     if (!slice_active(d)) return;
-    dual_workgroup<R, DUAL, false, NT>(d, ndual, nullptr);
+    dual_workgroup<R, DUAL, GAP, NT>(d, ndual, part_dual);
     return;
   }
   const int li = xcd_item(blockIdx.x, nitems);
@@ -2049,11 +2049,11 @@ __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double*
     for (int q = 1; q < ngs; ++q) w = w + other[q * W + xr];
     const C un = e_uo + tau * (e_div - w);
     const C ubn = (R(1) + theta) * un - theta * e_uo;
-    if (!NEXT || gi == 0) {
+    if (!NEXT || gi == 0) {  // (NEXT: one thread group stores and counts the pixel)
       gu[i] = un;
       gub[i] = ubn;
+      if (GAP) t[0] += dabs2(w - e_div);
     }
-    if (GAP) t[0] += dabs2(w - e_div);
     if constexpr (NEXT) {
       // the next forward row pass's input (what k_rows_fwd's row items form
       // from ubar, the same operations): S_c ubar, ifftshifted, into the
@@ -2120,16 +2120,17 @@ __device__ __forceinline__ void rows_adj_fused(const SliceDev& d, int y, double*
 // so the compiler's waits can leave the sensitivity loads, issued after the
 // T2 gather and the primal-step loads, in flight during the inverse FFT (the
 // other paths' runtime branches make it wait for every load before it).
-// NEXT (with FUSED, no gap evaluation): the row blocks also compute the
-// next iteration's T1 from their new ubar row, which k_rows_fwd's row items
-// would otherwise do (rows_adj_fused).
+// NEXT (with FUSED): the row blocks also compute the next iteration's T1
+// from their new ubar row, which k_rows_fwd's row items would otherwise do
+// (rows_adj_fused).  Its gap-evaluation variant (one iteration in
+// gap_every) is not held to 64 VGPRs: within them it spilled.
 template <class R, int MODE, bool GAP, int NT, int NF, bool FUSED = false, bool NEXT = false>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1024 && NF > 0 && ct_pad_ok(NF) ? 8 : 1))) void k_rows_adj(const SliceDev* sl, const int2* items, int nitems, int cb,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 1024 && NF > 0 && ct_pad_ok(NF) && !(GAP && NEXT) ? 8 : 1))) void k_rows_adj(const SliceDev* sl, const int2* items, int nitems, int cb,
                                                  double* part_rows, int nvstep) {
   extern __shared__ char smem_raw[];
   using C = Cx<R>;
   static_assert(!FUSED || (NF > 0 && MODE != M_OP && NT == kRowAdjThreadsWide), "fused adjoint row path");
-  static_assert(!NEXT || (FUSED && !GAP), "next forward rows: the fused path, solver iterations without the gap");
+  static_assert(!NEXT || FUSED, "next forward rows: the fused path");
   if (int(blockIdx.x) >= nitems) {
     // nvstep workgroups after the row items (a single slice): the v step of
     // TGV_primal_v_kernel on contiguous pixel ranges, where the row items
@@ -2769,13 +2770,13 @@ static void mri_stage_m(const Launch& L, bool gap, int stage) {
       const int2* items = L.dense ? nullptr : L.cols;
       if constexpr (NF > 0) {
         if (L.cg <= kColPrefetchCols) {
-          if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF, true>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1], 0);
-          else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF, true>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1], 0);
+          if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF, true>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1], 0, nullptr);
+          else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF, true>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1], 0, nullptr);
           return;
         }
       }
-      if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1], 0);
-      else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1], 0);
+      if (gap) PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, true, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1], 0, nullptr);
+      else PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF>), L.ncolitems, NC, lds, L.stream, L.sl, items, L.ncolitems, L.cg, L.part[1], 0, nullptr);
     });
   } else if (stage == 2) {
     with_fft_len<R>(L.fft_w, [&](auto nf) {
@@ -2797,36 +2798,45 @@ static void mri_stage(const Launch& L, int kind, bool gap, int stage) {
 
 // The two launches of a paired iteration (L.pair): 0 the column pass with
 // the dual step's workgroups, 1 the fused adjoint row pass that also writes
-// the next T1.
-template <class R, int MODE>
+// the next T1.  With the gap evaluation the partial sums go where the three
+// passes put them: the dual step's after the forward row items' (which a
+// paired iteration does not run; the engine zeroes theirs once), the column
+// items' and the row items' as in those passes.
+template <class R, int MODE, bool GAP>
 static void mri_pair_m(const Launch& L, int which) {
   constexpr int NC = kColThreads, NW = kRowAdjThreadsWide;
   if (which == 0) {
     const size_t lds = lds_occupancy(L.lds_cols, L.wpc[WPC_COLS], init_kernel_attrs());
+    double* const pd = L.part[0] + size_t(L.nrg) * kGapTerms;
     with_fft_len<R>(L.fft_h, [&](auto nf) {
       constexpr int NF = decltype(nf)::value;
       if constexpr (NF > 0) {
         if (L.cg <= kColPrefetchCols) {
-          PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF, true, MODE>), L.ncolitems + L.ndual, NC, lds, L.stream, L.sl, nullptr, L.ncolitems, L.cg, L.part[1], L.ndual);
+          PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, GAP, NC, NF, true, MODE>), L.ncolitems + L.ndual, NC, lds, L.stream, L.sl, nullptr, L.ncolitems, L.cg, L.part[1], L.ndual, pd);
           return;
         }
       }
-      PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, false, NC, NF, false, MODE>), L.ncolitems + L.ndual, NC, lds, L.stream, L.sl, nullptr, L.ncolitems, L.cg, L.part[1], L.ndual);
+      PDT_LAUNCH_NT((k_cols<R, CM_SOLVE, GAP, NC, NF, false, MODE>), L.ncolitems + L.ndual, NC, lds, L.stream, L.sl, nullptr, L.ncolitems, L.cg, L.part[1], L.ndual, pd);
     });
   } else {
     const size_t lds = lds_occupancy(L.lds_rows_a, L.wpc[WPC_ADJ], init_kernel_attrs());
     with_fft_len<R>(L.fft_w, [&](auto nf) {
       constexpr int NF = decltype(nf)::value;
       if constexpr (NF > 0)
-        PDT_LAUNCH_NT((k_rows_adj<R, MODE, false, NW, NF, true, true>), L.nrg_a + L.nvstep, NW, lds, L.stream, L.sl, nullptr, L.nrg_a, L.cb_a, L.gap_rows, L.nvstep);
+        PDT_LAUNCH_NT((k_rows_adj<R, MODE, GAP, NW, NF, true, true>), L.nrg_a + L.nvstep, NW, lds, L.stream, L.sl, nullptr, L.nrg_a, L.cb_a, L.gap_rows, L.nvstep);
     });
   }
 }
 
-void run_pair_stage(const Launch& L, int kind, int dtype, int which) {
+void run_pair_stage(const Launch& L, int kind, int dtype, int which, bool gap) {
   if (dtype != DT_C64) return;  // pairs need the compile-time row plans (single precision)
-  if (kind == K_TV_OP) mri_pair_m<float, M_TV>(L, which);
-  else mri_pair_m<float, M_TGV>(L, which);
+  if (kind == K_TV_OP) {
+    if (gap) mri_pair_m<float, M_TV, true>(L, which);
+    else mri_pair_m<float, M_TV, false>(L, which);
+  } else {
+    if (gap) mri_pair_m<float, M_TGV, true>(L, which);
+    else mri_pair_m<float, M_TGV, false>(L, which);
+  }
 }
 
 void run_stage(const Launch& L, int kind, int dtype, bool fp16, bool gap, int stage) {
@@ -2871,7 +2881,7 @@ static void op_forward(const Launch& L) {
   });
   with_fft_len<R>(L.fft_h, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;
-    PDT_LAUNCH_NT((k_cols<R, CM_FWD, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1], 0);
+    PDT_LAUNCH_NT((k_cols<R, CM_FWD, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1], 0, nullptr);
   });
 }
 
@@ -2880,7 +2890,7 @@ static void op_adjoint(const Launch& L) {
   constexpr int NC = kColThreads;
   with_fft_len<R>(L.fft_h, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;
-    PDT_LAUNCH_NT((k_cols<R, CM_ADJ, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1], 0);
+    PDT_LAUNCH_NT((k_cols<R, CM_ADJ, false, NC, NF>), L.ncolitems, NC, L.lds_cols, L.stream, L.sl, L.cols, L.ncolitems, L.cg, L.part[1], 0, nullptr);
   });
   with_fft_len<R>(L.fft_w, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;
@@ -2923,6 +2933,8 @@ static bool set_lds_attrs(int b) {
       if constexpr (std::is_same<R, float>::value) {
         set((const void*)&k_rows_adj<R, M_TV, false, kRowAdjThreadsWide, NF, true, true>);
         set((const void*)&k_rows_adj<R, M_TGV, false, kRowAdjThreadsWide, NF, true, true>);
+        set((const void*)&k_rows_adj<R, M_TV, true, kRowAdjThreadsWide, NF, true, true>);
+        set((const void*)&k_rows_adj<R, M_TGV, true, kRowAdjThreadsWide, NF, true, true>);
       }
     }
     set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF>);
@@ -2930,6 +2942,8 @@ static bool set_lds_attrs(int b) {
     if constexpr (std::is_same<R, float>::value) {
       set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF, false, M_TV>);
       set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF, false, M_TGV>);
+      set((const void*)&k_cols<R, CM_SOLVE, true, NC, NF, false, M_TV>);
+      set((const void*)&k_cols<R, CM_SOLVE, true, NC, NF, false, M_TGV>);
     }
     if constexpr (NF > 0) {
       set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF, true>);
@@ -2937,6 +2951,8 @@ static bool set_lds_attrs(int b) {
       if constexpr (std::is_same<R, float>::value) {
         set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF, true, M_TV>);
         set((const void*)&k_cols<R, CM_SOLVE, false, NC, NF, true, M_TGV>);
+        set((const void*)&k_cols<R, CM_SOLVE, true, NC, NF, true, M_TV>);
+        set((const void*)&k_cols<R, CM_SOLVE, true, NC, NF, true, M_TGV>);
       }
     }
     set((const void*)&k_cols<R, CM_FWD, false, NC, NF>);

@@ -91,12 +91,12 @@ void run_iteration(const Launch& L, int kind, int dtype, bool fp16, bool gap);
 int num_stages(int kind);
 const char* stage_name(int kind, int stage);
 void run_stage(const Launch& L, int kind, int dtype, bool fp16, bool gap, int stage);
-// One launch of a paired MRI iteration (L.pair, no gap evaluation): which 0
-// = the column pass with the dual step (reads the T1 of the current ubar),
-// 1 = the adjoint row pass with the primal step that also writes the T1 of
-// the new ubar.  The first iteration of a run of pairs starts from stages 0
-// and 1, and the last ends with stage 2 (which writes no T1).
-void run_pair_stage(const Launch& L, int kind, int dtype, int which);
+// One launch of a paired MRI iteration (L.pair): which 0 = the column pass
+// with the dual step (reads the T1 of the current ubar), 1 = the adjoint row
+// pass with the primal step that also writes the T1 of the new ubar; gap:
+// the gap-evaluation variants (partial sums where the three passes put them).
+// The first iteration of a solve starts from stages 0 and 1 instead of 0.
+void run_pair_stage(const Launch& L, int kind, int dtype, int which, bool gap);
 // tb_k ROF or TGV (kind) iterations of a single dense slice in one launch
 // (tiles held in LDS with a tb_k-pixel halo), reading the state (ROF: u,
 // ubar, p; TGV: u, v, p, q) and writing its second copy with ubar (and vbar)
