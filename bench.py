@@ -253,6 +253,16 @@ def _timed_solves(opt, u0_dev, steps: int) -> float:
     return float(np.median(ts))
 
 
+def _equivalent(one_iteration_bytes: float, us_per_iteration: float) -> dict:
+    """The bytes the one-iteration kernels move per iteration (each plane
+    read or written once per half-iteration) over the measured time per
+    iteration: what a temporally blocked solve does in HBM-roofline terms
+    (its own launches move those bytes once per k iterations)."""
+    gbs = one_iteration_bytes / (us_per_iteration * 1e-6) / 1e9
+    return {"one_iteration_MB": round(one_iteration_bytes / 1e6, 3), "GBs": round(gbs, 1),
+            "frac_of_peak": round(gbs / HBM_PEAK_GBS, 3)}
+
+
 def _kernels(prof) -> dict:
     """Per-launch duration (ms), algorithmic bytes (MB) and the GB/s they
     give, per kernel of one iteration (pdt_solver_profile)."""
@@ -286,7 +296,9 @@ def extra_tv256(steps: int, cpu_threads: int, no_cpu: bool, autotune: bool = Tru
     out = {"workload": "TV-L2 (ROF) 256x256 float32, lambda=10, 500 iterations",
            "seconds": s, "us_per_iteration": s * 1e6 / 500, "rel_gap": float(st["rel_gap"]),
            "rel_l2_vs_f64": _rel(u, res["f64"][2]), "seconds_f64": res["f64"][1], "tiles": tiles,
-           "kernels": _kernels(opt.profileKernels(50))}
+           "kernels": _kernels(opt.profileKernels(50)),
+           # dual step: ubar r, p r/w; primal step: u r/w, f, p r, ubar w (4 bytes each)
+           "equivalent": _equivalent((5 + 6) * 4.0 * 256 * 256, s * 1e6 / 500)}
     if not no_cpu:
         O = _oracle()
         O.set_threads(cpu_threads)
@@ -323,9 +335,12 @@ def extra_tgv512(steps: int, cpu_threads: int, no_cpu: bool, autotune: bool = Tr
     out = {"workload": "TGV2-L2 512x512 complex64, lambda=20, alpha0=2, alpha1=1, 1000 iterations"}
     for name in ("f32", "f16dual"):
         opt, s, u = res[name]
+        # primal step 10 pe + 5 pq, dual step 3 pe + 10 pq bytes per pixel
+        pq = 4.0 if name == "f16dual" else 8.0
         out[name] = {"seconds": s, "us_per_iteration": s * 1e6 / 1000, "rel_l2_vs_f64": _rel(u, ref),
                      "rel_gap": float(opt.getStats()["rel_gap"]), "tiles": tiles.get(name),
-                     "kernels": _kernels(opt.profileKernels(50))}
+                     "kernels": _kernels(opt.profileKernels(50)),
+                     "equivalent": _equivalent((13 * 8.0 + 15 * pq) * 512 * 512, s * 1e6 / 1000)}
     out["seconds_f64"] = res["f64"][1]
     if not no_cpu:
         O = _oracle()
