@@ -970,8 +970,8 @@ __global__ __launch_bounds__(NT) void k_tv_primal(const SliceDev* sl, const int2
 
 // Temporally blocked ROF iterations: tb_k times TV_dual_p_kernel then
 // TV_primal_u_kernel (tgv_kernels.cuh:294-324,568-605) per launch, on a tile
-// held in LDS.  Block (bx, by) owns the interior [x0, x1) x [y0, y1) and
-// loads u, ubar, p (2 planes) and f on the interior grown by k pixels on
+// held by the workgroup.  Block (bx, by) owns the interior [x0, x1) x [y0, y1)
+// and loads u, ubar, p (2 planes) and f on the interior grown by k pixels on
 // every side (clipped to the image).  Each iteration shrinks the region
 // where the values are exact by one pixel on each side that is not an image
 // border: the dual step at (x, y) needs ubar at x + 1 and y + 1 (forward
@@ -979,17 +979,18 @@ __global__ __launch_bounds__(NT) void k_tv_primal(const SliceDev* sl, const int2
 // y - 1 (backward differences, none at the first).  After k iterations the
 // interior is exact and is written to the other copy of the state (src ->
 // dst), so that no block reads a halo that another block of the same launch
-// writes.  The per-pixel arithmetic is that of k_tv_dual / k_tv_primal, in
-// the same order, so the iterates equal theirs bit for bit.
-// Each thread of k_tv_block takes its pixels of a phase in batches of
-// kTvBlockBatch, all loads of a batch issued before its arithmetic, so that
-// the LDS latencies of a batch overlap.
-constexpr int kTvBlockBatch = 4;
+// writes.  Each thread owns up to tv_block_own pixels of the tile for the
+// whole launch and keeps their u, f and p in registers (only the pixel
+// itself reads u and f, and its own p in the dual step); ubar and p, which
+// neighbours read, are the three LDS planes.  The per-pixel arithmetic is
+// that of k_tv_dual / k_tv_primal, in the same order, so the iterates equal
+// theirs bit for bit.
+constexpr int tv_block_own(size_t) { return 4; }  // (block_threads: 4 pixels per thread up to 4096)
 
 template <class P, int NT>
 __global__ __launch_bounds__(NT) void k_tv_block(const SliceDev* sl, int tx, int ty, int k, int dir) {
   extern __shared__ char smem_raw[];
-  constexpr int B = kTvBlockBatch;
+  constexpr int J = tv_block_own(sizeof(P));
   const SliceDev& d = sl[0];
   if (!slice_active(d)) return;
   using R = typename RealOf<P>::T;
@@ -999,44 +1000,45 @@ __global__ __launch_bounds__(NT) void k_tv_block(const SliceDev* sl, int tx, int
   const int by = b / nbx, bx = b - by * nbx;
   const int x0 = bx * tx, x1 = min(W, x0 + tx), y0 = by * ty, y1 = min(H, y0 + ty);
   const int xa = max(0, x0 - k), xb = min(W, x1 + k), ya = max(0, y0 - k), yb = min(H, y1 + k);
-  const int LW = xb - xa, LN = LW * (yb - ya);
-  P* sU = reinterpret_cast<P*>(smem_raw);
-  P* sB = sU + LN;
+  const int LW = xb - xa, LN = LW * (yb - ya);  // LN <= J * NT (block_fits)
+  P* sB = reinterpret_cast<P*>(smem_raw);
   P* sX = sB + LN;
   P* sY = sX + LN;
-  P* sF = sY + LN;
   const size_t n = size_t(W) * H;
+  // the owned pixels e = threadIdx.x + j NT of the tile (x = -1: none)
+  P ru[J], rf[J], rpx[J], rpy[J];
+  int rx[J], ry[J];
   {
     const auto gu = gptr<const P>(dir ? d.u2 : d.u);
     const auto gb = gptr<const P>(dir ? d.ub2 : d.ub);
     const auto gp = gptr<const P>(dir ? d.p2 : d.p);
     const auto gf = gptr<const P>(d.f);
     const float rLW = 1.0f / float(LW);
-    for (int e0 = threadIdx.x; e0 < LN; e0 += NT * B) {
-      P vu[B], vb[B], vx[B], vy[B], vf[B];
+    P vb[J];
 #pragma unroll
-      for (int j = 0; j < B; ++j) {
-        const int e = e0 + j * NT;
-        if (e < LN) {
-          const int ly = qdiv(e, LW, rLW), lx = e - ly * LW;
-          const size_t gi = size_t(ya + ly) * W + (xa + lx);
-          vu[j] = gu[gi];
-          vb[j] = gb[gi];
-          vx[j] = gp[gi];
-          vy[j] = gp[gi + n];
-          vf[j] = gf[gi];
-        }
+    for (int j = 0; j < J; ++j) {
+      const int e = threadIdx.x + j * NT;
+      rx[j] = -1;
+      ry[j] = -1;
+      if (e < LN) {
+        const int ly = qdiv(e, LW, rLW), lx = e - ly * LW;
+        rx[j] = xa + lx;
+        ry[j] = ya + ly;
+        const size_t gi = size_t(ry[j]) * W + rx[j];
+        ru[j] = gu[gi];
+        vb[j] = gb[gi];
+        rpx[j] = gp[gi];
+        rpy[j] = gp[gi + n];
+        rf[j] = gf[gi];
       }
+    }
 #pragma unroll
-      for (int j = 0; j < B; ++j) {
-        const int e = e0 + j * NT;
-        if (e < LN) {
-          sU[e] = vu[j];
-          sB[e] = vb[j];
-          sX[e] = vx[j];
-          sY[e] = vy[j];
-          sF[e] = vf[j];
-        }
+    for (int j = 0; j < J; ++j) {
+      const int e = threadIdx.x + j * NT;
+      if (e < LN) {
+        sB[e] = vb[j];
+        sX[e] = rpx[j];
+        sY[e] = rpy[j];
       }
     }
   }
@@ -1046,37 +1048,21 @@ __global__ __launch_bounds__(NT) void k_tv_block(const SliceDev* sl, int tx, int
   for (int it = 0; it < k; ++it) {
     // dual step where ubar and its forward neighbours are exact
     const int pxa = uxa, pxb = uxb == W ? W : uxb - 1, pya = uya, pyb = uyb == H ? H : uyb - 1;
-    {
-      const int rw = pxb - pxa, cnt = rw * (pyb - pya);
-      const float rr = 1.0f / float(max(rw, 1));
-      for (int e0 = threadIdx.x; e0 < cnt; e0 += NT * B) {
-        int l[B];
-        P c[B], ex[B], ey[B], ox[B], oy[B];
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-          const int e = e0 + j * NT;
-          if (e < cnt) {
-            const int ry = qdiv(e, rw, rr), x = pxa + (e - ry * rw), y = pya + ry;
-            l[j] = (y - ya) * LW + (x - xa);
-            c[j] = sB[l[j]];
-            ex[j] = sB[l[j] + (x < W - 1)];
-            ey[j] = sB[l[j] + ((y < H - 1) ? LW : 0)];
-            ox[j] = sX[l[j]];
-            oy[j] = sY[l[j]];
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < B; ++j) {
-          if (e0 + j * NT < cnt) {
-            const P tx_ = ex[j] - c[j], ty_ = ey[j] - c[j];
-            const P ax = ox[j] + sigma * tx_;
-            const P ay = oy[j] + sigma * ty_;
-            const R pn = sqrt(abs2(ax) + abs2(ay));
-            const R sp = rcp(fmax(R(1), pn));
-            sX[l[j]] = ax * sp;
-            sY[l[j]] = ay * sp;
-          }
-        }
+    for (int j = 0; j < J; ++j) {
+      const int x = rx[j], y = ry[j];
+      if (x >= pxa && x < pxb && y >= pya && y < pyb) {
+        const int l = threadIdx.x + j * NT;
+        const P c = sB[l], ex = sB[l + (x < W - 1)], ey = sB[l + ((y < H - 1) ? LW : 0)];
+        const P tx_ = ex - c, ty_ = ey - c;
+        const P ax = rpx[j] + sigma * tx_;
+        const P ay = rpy[j] + sigma * ty_;
+        const R pn = sqrt(abs2(ax) + abs2(ay));
+        const R sp = rcp(fmax(R(1), pn));
+        rpx[j] = ax * sp;
+        rpy[j] = ay * sp;
+        sX[l] = rpx[j];
+        sY[l] = rpy[j];
       }
     }
     __syncthreads();
@@ -1085,36 +1071,21 @@ __global__ __launch_bounds__(NT) void k_tv_block(const SliceDev* sl, int tx, int
     uxb = pxb;
     uya = pya == 0 ? 0 : pya + 1;
     uyb = pyb;
-    {
-      const int rw = uxb - uxa, cnt = rw * (uyb - uya);
-      const float rr = 1.0f / float(max(rw, 1));
-      for (int e0 = threadIdx.x; e0 < cnt; e0 += NT * B) {
-        int l[B];
-        P div[B], uo[B], ff[B];
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-          const int e = e0 + j * NT;
-          if (e < cnt) {
-            const int ry = qdiv(e, rw, rr), x = uxa + (e - ry * rw), y = uya + ry;
-            l[j] = (y - ya) * LW + (x - xa);
-            const P cx = sX[l[j]], mx = sX[x > 0 ? l[j] - 1 : l[j]];
-            const P dx = (x < W - 1) ? cx - mx : -mx;
-            const P cy = sY[l[j]], my = sY[y > 0 ? l[j] - LW : l[j]];
-            const P dy = (y < H - 1) ? cy - my : -my;
-            div[j] = (x > 0 ? dx : cx) + (y > 0 ? dy : cy);
-            uo[j] = sU[l[j]];
-            ff[j] = sF[l[j]];
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < B; ++j) {
-          if (e0 + j * NT < cnt) {
-            P un = uo[j] + tau * fmadd(lambda, ff[j], div[j]);
-            un = un * rcp(R(1) + tau * lambda);
-            sU[l[j]] = un;
-            sB[l[j]] = (R(1) + theta) * un - theta * uo[j];
-          }
-        }
+    for (int j = 0; j < J; ++j) {
+      const int x = rx[j], y = ry[j];
+      if (x >= uxa && x < uxb && y >= uya && y < uyb) {
+        const int l = threadIdx.x + j * NT;
+        const P cx = rpx[j], mx = sX[x > 0 ? l - 1 : l];
+        const P dx = (x < W - 1) ? cx - mx : -mx;
+        const P cy = rpy[j], my = sY[y > 0 ? l - LW : l];
+        const P dy = (y < H - 1) ? cy - my : -my;
+        const P div = (x > 0 ? dx : cx) + (y > 0 ? dy : cy);
+        const P uo = ru[j];
+        P un = uo + tau * fmadd(lambda, rf[j], div);
+        un = un * rcp(R(1) + tau * lambda);
+        ru[j] = un;
+        sB[l] = (R(1) + theta) * un - theta * uo;
       }
     }
     __syncthreads();
@@ -1122,16 +1093,17 @@ __global__ __launch_bounds__(NT) void k_tv_block(const SliceDev* sl, int tx, int
   const auto ou = gptr<P>(dir ? d.u : d.u2);
   const auto ob = gptr<P>(dir ? d.ub : d.ub2);
   const auto op = gptr<P>(dir ? d.p : d.p2);
-  const int iw = x1 - x0, icnt = iw * (y1 - y0);
-  const float riw = 1.0f / float(iw);
-  for (int e = threadIdx.x; e < icnt; e += NT) {
-    const int ry = qdiv(e, iw, riw), x = x0 + (e - ry * iw), y = y0 + ry;
-    const int l = (y - ya) * LW + (x - xa);
-    const size_t gi = size_t(y) * W + x;
-    ou[gi] = sU[l];
-    ob[gi] = sB[l];
-    op[gi] = sX[l];
-    op[gi + n] = sY[l];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int x = rx[j], y = ry[j];
+    if (x >= x0 && x < x1 && y >= y0 && y < y1) {
+      const int l = threadIdx.x + j * NT;
+      const size_t gi = size_t(y) * W + x;
+      ou[gi] = ru[j];
+      ob[gi] = sB[l];
+      op[gi] = rpx[j];
+      op[gi + n] = rpy[j];
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) gptr<int>(d.ctl)[CTL_ITER] += k;
 }
@@ -2659,25 +2631,27 @@ static void with_fft_len(int n, Fn&& fn) {
 
 size_t block_lds(int kind, int dtype, int tx, int ty, int k) {
   const size_t e = dtype == DT_F32 ? 4 : (dtype == DT_C128 ? 16 : 8);
-  const size_t planes = kind == K_TGV ? 8 : 5;  // TGV: ubar, vbar, p, q; ROF: u, ubar, p, f
+  const size_t planes = kind == K_TGV ? 8 : 3;  // TGV: ubar, vbar, p, q; ROF: ubar, p
   return planes * size_t(tx + 2 * k) * size_t(ty + 2 * k) * e;
 }
 
-// Workgroup size of a blocked launch: about one batch of pixels per thread of
-// the loaded tile (ROF: kTvBlockBatch pixels, TGV: one to tgv_block_own),
+// Workgroup size of a blocked launch: a few owned pixels per thread of the
+// loaded tile (ROF: up to 4 below 4096 pixels, TGV: one to tgv_block_own),
 // 256 to 1024 threads.
 static int block_threads(int kind, int dtype, int tx, int ty, int k) {
   const int px = (tx + 2 * k) * (ty + 2 * k);
   if (kind == K_TGV) return px >= 1024 && dtype != DT_C128 ? 1024 : px >= 384 ? 512 : 256;
-  return px > 2048 * kTvBlockBatch / 2 ? 1024 : px > 256 * kTvBlockBatch ? 512 : 256;
+  return px > 2048 ? 1024 : px > 1024 ? 512 : 256;
 }
 
 bool block_fits(int kind, int dtype, int tx, int ty, int k, size_t lds_limit) {
   if (tx <= 0 || ty <= 0 || k <= 0) return false;
   const int px = (tx + 2 * k) * (ty + 2 * k);
-  // TGV: every tile pixel owned by a thread (tgv_block_own per thread at most)
+  // every tile pixel owned by a thread (TGV: tgv_block_own per thread at most)
   const size_t e = dtype == DT_F32 ? 4 : (dtype == DT_C128 ? 16 : 8);
   if (kind == K_TGV && px > tgv_block_own(e) * block_threads(kind, dtype, tx, ty, k)) return false;
+  // ROF: likewise, tv_block_own pixels per thread at most
+  if (kind != K_TGV && px > tv_block_own(e) * block_threads(kind, dtype, tx, ty, k)) return false;
   return block_lds(kind, dtype, tx, ty, k) <= lds_limit;
 }
 
