@@ -252,7 +252,9 @@ PDT_API int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* o
 
 /* NEW: a manual tile choice for the shape of these solvers (one batch),
  * stored in the same tile cache as pdt_autotune's result: cb, cg, rpi, cb_a
- * <= 0 take the default, values above a shape's limit are clamped; wpc in
+ * <= 0 take the default (cb_a: every coil of a row in one adjoint workgroup
+ * where a single slice's straight-line adjoint path fits, else cb), values
+ * above a shape's limit are clamped; wpc in
  * [0, 6]; tb_k <= 0 turns the blocked denoising launches off, and tiles
  * that do not fit the LDS are not used.  The next solve of the same shape
  * uses it. */

@@ -801,7 +801,7 @@ struct Engine {
   bool fp16 = false;
   bool pair_duals = true;  // TGV denoising: the dual step as 2-pixel vectors (even widths)
   int force_dpx = 0;       // dual step pixels per thread (0: cache / default)
-  int force_cb = 0, force_cg = 0, force_rpi = 0, force_cb_a = 0;  // autotuner overrides (0: cache / default)
+  int force_cb = 0, force_cg = 0, force_rpi = 0, force_cb_a = 0;  // autotuner overrides (0: cache / default; cb_a -1: default)
   int force_wpc[4] = {-1, -1, -1, -1};  // occupancy caps (-1: cache; 0: none)
   int force_tb[3] = {0, 0, -1};         // blocked ROF tiles (tb[2] -1: cache / default; 0: off)
   int force_unpaired = -1;              // MRI: 1 three launches per iteration, 0 pairs where possible (-1: cache)
@@ -878,7 +878,13 @@ struct Engine {
       unpaired = force_unpaired >= 0 ? force_unpaired != 0 : tv.unpaired != 0;
       cb = force_cb > 0 ? force_cb : tv.cb > 0 ? tv.cb : kDefaultCb;
       cg = force_cg > 0 ? force_cg : tv.cg > 0 ? tv.cg : kDefaultCg;
-      cb_a = force_cb_a > 0 ? force_cb_a : tv.cb_a > 0 ? tv.cb_a : cb;
+      // default adjoint grouping: every coil of a row in one wide block where
+      // a single slice's straight-line adjoint path fits (with the u and v
+      // steps in that pass and, by default, paired iterations), else cb
+      const bool fused_fits = ns == 1 && sv[0]->op->kind != PDT_OP_SAMPLING &&
+                              maxC * maxW >= 8 * kRowThreads && rows_adj_fused_ok(dt, maxW, maxC, maxNc);
+      const int def_cb_a = fused_fits ? maxC : cb;
+      cb_a = force_cb_a > 0 ? force_cb_a : force_cb_a < 0 ? def_cb_a : tv.cb_a > 0 ? tv.cb_a : def_cb_a;
       cb = std::max(1, std::min(cb, max_cb(dt, maxW, maxNc, maxC)));
       cb_a = std::max(1, std::min(cb_a, max_cb(dt, maxW, maxNc, maxC)));
       cg = std::max(1, std::min(cg, max_cg(dt, maxHk, maxNc)));
@@ -2396,7 +2402,7 @@ int pdt_set_tiles(pdt_solver** solvers, int count, const pdt_tiles* t) {
     e.force_cb = t->cb > 0 ? t->cb : kDefaultCb;
     e.force_cg = t->cg > 0 ? t->cg : kDefaultCg;
     e.force_rpi = t->rpi > 0 ? t->rpi : 1;
-    e.force_cb_a = t->cb_a > 0 ? t->cb_a : e.force_cb;
+    e.force_cb_a = t->cb_a > 0 ? t->cb_a : -1;  // (-1: the engine's default grouping)
     for (int i = 0; i < 4; ++i) e.force_wpc[i] = std::max(0, std::min(t->wpc[i], 6));
     e.force_tb[0] = t->tb_x;
     e.force_tb[1] = t->tb_y;
