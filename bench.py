@@ -31,6 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "primal-dual-toolbox_amd"))
 HBM_PEAK_GBS = 8000.0
 GAP_TOL = 1e-4
 GAP_EVERY = 50
+CONC_STREAMS = 4  # slices solved at once by pdt_solve_concurrent in the batch leg
 
 
 def log(*a):
@@ -473,6 +474,19 @@ def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool, auto
         alone_ms.append(float(o.getStats()["ms"]))
     seq_s = time.perf_counter() - t0
     log(f"[bench] batch: sequential {seq_s:.3f} s")
+    # the same single-slice solves, CONC_STREAMS at a time on their own streams
+    seq_u = [o.getResult() for o in solvers]
+    conc_times = []
+    for r in range(runs):
+        cold()
+        t0 = time.perf_counter()
+        pdt.mri.solve_concurrent(solvers, streams=CONC_STREAMS)
+        conc_times.append(time.perf_counter() - t0)
+        log(f"[bench] batch: concurrent run {r}: {conc_times[-1]:.3f} s")
+    conc_its = [int(o.getStats()["iters"]) for o in solvers]
+    if conc_its != its or not all(np.array_equal(o.getResult(), u) for o, u in zip(solvers, seq_u)):
+        raise RuntimeError("concurrent solves differ from the single solves")
+    conc_s = float(np.median(conc_times))
     work = [it * c["meta"]["coils"] * 2 * c["meta"]["n"] * c["meta"]["n_pe"] for it, c in zip(its, cases)]
 
     def dist(x):
@@ -482,6 +496,8 @@ def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool, auto
                         "rel. gap 1e-4 per slice, cold start"),
            "seconds": batch_s, "seconds_runs": times, "slices_per_s": count / batch_s,
            "sequential_seconds": seq_s, "sequential_slices_per_s": count / seq_s,
+           "concurrent": {"streams": CONC_STREAMS, "seconds": conc_s, "seconds_runs": conc_times,
+                          "slices_per_s": count / conc_s, "equal_to_single_solves": True},
            "iterations": {k: int(v) for k, v in dist(its).items()},
            "time_to_gap_alone_ms": dist(alone_ms),
            "time_to_stop_in_batch_ms": dist(stop_ms),

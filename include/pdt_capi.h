@@ -265,6 +265,19 @@ PDT_API int pdt_set_tiles(pdt_solver** solvers, int count, const pdt_tiles* tile
  * solver keeps its own parameters, sizes, coils and stopping. */
 PDT_API int pdt_batch_solve(pdt_solver** solvers, int count, int verbose);
 
+/* NEW (no counterpart in the reference, which solves one problem per call,
+ * tgvmrioptimizer.cu:94-203): solves count configured solvers, each alone
+ * as pdt_solver_solve would, `streams` of them at a time (1..16): host
+ * worker threads (the calling thread among them) take the next unsolved
+ * solver in the given order and run its solve on the solver's own HIP
+ * stream, so the launches of different slices overlap on the GPU.  Solvers
+ * may differ in kind, dtype, size and parameters; each one's iterates, stop
+ * and statistics are those of its own pdt_solver_solve.  Null or repeated
+ * solvers are refused before any solve starts; after a failure no further
+ * solver is started and the first error is returned. */
+#define PDT_MAX_CONCURRENT 16
+PDT_API int pdt_solve_concurrent(pdt_solver** solvers, int count, int streams, int verbose);
+
 /* ---- NEW: learned warm start ----------------------------------------------
  * A conv3x3 network (zero padding, stride 1, ReLU between layers) that maps
  * the zero-filled reconstruction A^*f -- 2 channels (re, im) divided by its

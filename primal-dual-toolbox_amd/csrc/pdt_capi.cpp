@@ -20,6 +20,7 @@
 #include <random>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pdt_capi.h"
@@ -35,6 +36,17 @@ using namespace pdt;
 // ---------------------------------------------------------------------------
 namespace {
 thread_local std::string g_err;
+
+// Held by every graph capture and by the host set-up that issues synchronous
+// operations on the legacy stream (engine builds, solve preparation,
+// operator set-up and applications, pinned-buffer allocation and release,
+// work-list uploads): HIP refuses a legacy-stream operation that would
+// depend on a stream another thread is capturing, so solves on several host
+// threads (pdt_solve_concurrent) never run the two at once.  The operator's
+// applications, which use its own descriptor and intermediates, also run one
+// at a time under it.
+std::recursive_mutex g_setup_mu;
+using SetupLock = std::lock_guard<std::recursive_mutex>;
 
 struct Err {
   int code;
@@ -338,6 +350,7 @@ struct pdt_op {
   }
 
   void prepare() {
+    SetupLock lk(g_setup_mu);
     check_count();
     if (prepared) return;
     const HostConst& m = mask();
@@ -463,6 +476,7 @@ struct pdt_op {
 
   // Applies A (forward) or A^* (adjoint) from device src to device dst.
   void apply(bool forward, const void* src, void* dst, hipStream_t st) {
+    SetupLock lk(g_setup_mu);
     ensure_opmode();
     SliceDev d = geometry();
     d.ngroups = (ncols + cg - 1) / cg;
@@ -840,6 +854,7 @@ struct Engine {
   std::vector<GraphEntry> graphs;
 
   ~Engine() {
+    SetupLock lk(g_setup_mu);
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (h_cs) (void)hipHostFree(h_cs);
     for (int q = 0; q < 2; ++q) {
@@ -849,6 +864,7 @@ struct Engine {
   }
 
   void build() {
+    SetupLock lk(g_setup_mu);
     pdt_solver* s0 = sv[0];
     kind = s0->kind;
     dtype = s0->dtype;
@@ -1227,6 +1243,7 @@ struct Engine {
       }
     }
     if (items == 0 || 6 * items > 5 * listed_items) return;
+    SetupLock lk(g_setup_mu);
     std::vector<int2> rows, rgs, rgs_a, cols;
     std::vector<GroupRange> gr;
     make_lists(use, rows, rgs, rgs_a, cols, gr);
@@ -1383,6 +1400,7 @@ struct Engine {
         HIPC(hipGraphLaunch(g.exec, stream));
         return;
       }
+    SetupLock lk(g_setup_mu);
     hipGraph_t graph;
     HIPC(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     enqueue(len, gap, first);
@@ -1453,6 +1471,7 @@ struct Engine {
         return last_rel * std::min(1.0, last_rel / prev_rel) <= 1.02 * tol;
       };
       for (int q = 0; q < 2; ++q) {
+        SetupLock lk(g_setup_mu);
         if (!h_cs_q[q]) HIPC(hipHostMalloc((void**)&h_cs_q[q], cs_bytes, hipHostMallocDefault));
         if (!ev_q[q]) HIPC(hipEventCreateWithFlags(&ev_q[q], hipEventDisableTiming));
       }
@@ -1549,6 +1568,7 @@ void check_solvable(pdt_solver* s) {
 }
 
 void prepare_solve(pdt_solver* s, int verbose) {
+  SetupLock lk(g_setup_mu);
   check_solvable(s);
   if (!s->stream) HIPC(hipStreamCreate(&s->stream));
   if (s->is_op()) {
@@ -1605,20 +1625,26 @@ void solve_many(std::vector<pdt_solver*>& sv, int verbose) {
   for (auto* s : sv) prepare_solve(s, verbose);
   if (sv.size() == 1) {
     // reuse the engine of the previous solve when nothing it depends on
-    // changed: its work lists and instantiated graphs stay valid
+    // changed: its work lists and instantiated graphs stay valid (a replaced
+    // engine is released, and the new one built, under the set-up lock)
     pdt_solver* s = sv[0];
-    Engine probe;
-    probe.sv = sv;
-    probe.plan_only = true;
-    probe.build();
-    auto cached = std::static_pointer_cast<Engine>(s->engine);
-    if (!cached || cached->sig != probe.sig) {
-      s->engine.reset();
-      auto e = std::make_shared<Engine>();
-      e->sv = sv;
-      e->build();
-      s->engine = e;
-      cached = e;
+    std::shared_ptr<Engine> cached;
+    {
+      SetupLock lk(g_setup_mu);
+      Engine probe;
+      probe.sv = sv;
+      probe.plan_only = true;
+      probe.build();
+      cached = std::static_pointer_cast<Engine>(s->engine);
+      if (!cached || cached->sig != probe.sig) {
+        cached.reset();
+        s->engine.reset();
+        auto e = std::make_shared<Engine>();
+        e->sv = sv;
+        e->build();
+        s->engine = e;
+        cached = e;
+      }
     }
     cached->run(verbose);
     return;
@@ -2078,15 +2104,18 @@ void profile_engine(std::vector<pdt_solver*> sv, int reps, pdt_kernel_prof* out,
       HIPC(hipEventCreate(&e0));
       HIPC(hipEventCreate(&e1));
       hipGraph_t graph;
-      HIPC(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
-      for (int r = 0; r < reps; ++r) {
-        run_block(e.G[0], e.kind, e.dtk(), e.fp16, 0);
-        run_block(e.G[0], e.kind, e.dtk(), e.fp16, 1);
-      }
-      HIPC(hipStreamEndCapture(e.stream, &graph));
       hipGraphExec_t ge;
-      HIPC(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(graph);
+      {
+        SetupLock lk(g_setup_mu);
+        HIPC(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+        for (int r = 0; r < reps; ++r) {
+          run_block(e.G[0], e.kind, e.dtk(), e.fp16, 0);
+          run_block(e.G[0], e.kind, e.dtk(), e.fp16, 1);
+        }
+        HIPC(hipStreamEndCapture(e.stream, &graph));
+        HIPC(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+      }
       HIPC(hipGraphLaunch(ge, e.stream));
       HIPC(hipEventRecord(e0, e.stream));
       HIPC(hipGraphLaunch(ge, e.stream));
@@ -2124,13 +2153,16 @@ void profile_engine(std::vector<pdt_solver*> sv, int reps, pdt_kernel_prof* out,
     HIPC(hipEventCreate(&e0));
     HIPC(hipEventCreate(&e1));
     hipGraph_t graph;
-    HIPC(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
-    for (int r = 0; r < reps; ++r)
-      for (int st = 0; st < ns; ++st) e.iter_launch(st);
-    HIPC(hipStreamEndCapture(e.stream, &graph));
     hipGraphExec_t ge;
-    HIPC(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(graph);
+    {
+      SetupLock lk(g_setup_mu);
+      HIPC(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+      for (int r = 0; r < reps; ++r)
+        for (int st = 0; st < ns; ++st) e.iter_launch(st);
+      HIPC(hipStreamEndCapture(e.stream, &graph));
+      HIPC(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+    }
     HIPC(hipGraphLaunch(ge, e.stream));  // warm-up replay
     HIPC(hipEventRecord(e0, e.stream));
     HIPC(hipGraphLaunch(ge, e.stream));
@@ -2192,6 +2224,8 @@ int pdt_batch_profile(pdt_solver** solvers, int count, int reps, pdt_kernel_prof
   });
 }
 
+// helpers of the C ABI below, with C++ linkage
+extern "C++" {
 namespace {
 // Device time of each stage kernel of an iteration (ms per launch), each
 // stage timed over `reps` back-to-back launches after one untimed launch.
@@ -2243,6 +2277,7 @@ double time_iteration(Engine& e, int reps) {
   return tot;
 }
 }  // namespace
+}  // extern "C++"
 
 int pdt_autotune(pdt_solver** solvers, int count, int reps, pdt_tiles* out, double* out_ms) {
   return guard([&] {
@@ -2427,6 +2462,54 @@ int pdt_batch_solve(pdt_solver** solvers, int count, int verbose) {
     if (!solvers || count <= 0) fail(PDT_ERR_ARG, "empty batch");
     std::vector<pdt_solver*> v(solvers, solvers + count);
     solve_many(v, verbose);
+  });
+}
+
+int pdt_solve_concurrent(pdt_solver** solvers, int count, int streams, int verbose) {
+  return guard([&] {
+    if (!solvers || count <= 0) fail(PDT_ERR_ARG, "empty batch");
+    if (streams < 1 || streams > PDT_MAX_CONCURRENT) fail(PDT_ERR_ARG, "streams must be in [1, 16]");
+    std::vector<pdt_solver*> v(solvers, solvers + count);
+    for (int i = 0; i < count; ++i) {
+      if (!v[i]) fail(PDT_ERR_ARG, "null solver in batch");
+      for (int j = 0; j < i; ++j)
+        if (v[j] == v[i]) fail(PDT_ERR_ARG, "a solver appears twice in the batch");
+    }
+    // Workers (the calling thread and streams - 1 more) take the next
+    // unsolved solver in order and solve it alone on its own stream; the
+    // first failure stops the others from taking more and is reported here.
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
+    std::mutex mu;
+    int err_code = PDT_OK;
+    std::string err_msg;
+    auto work = [&] {
+      const int rc = guard([&] {
+        HIPC(hipSetDevice(dev));
+        for (;;) {
+          if (failed.load()) return;
+          const int i = next.fetch_add(1);
+          if (i >= count) return;
+          std::vector<pdt_solver*> one{v[i]};
+          solve_many(one, verbose);
+        }
+      });
+      if (rc != PDT_OK) {
+        failed.store(true);
+        std::lock_guard<std::mutex> lk(mu);
+        if (err_code == PDT_OK) {
+          err_code = rc;
+          err_msg = g_err;
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < std::min(streams, count); ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    if (err_code != PDT_OK) fail(err_code, err_msg);
   });
 }
 

@@ -29,7 +29,7 @@ EXPORTS = [
     "pdt_solver_create", "pdt_params_default", "pdt_solver_get_params", "pdt_solver_set_params",
     "pdt_solver_set_operator", "pdt_solver_set_noisy_data", "pdt_solver_set_input0",
     "pdt_solver_solve", "pdt_solver_get_result", "pdt_solver_get_v", "pdt_solver_get_stats",
-    "pdt_solver_destroy", "pdt_solver_profile", "pdt_batch_profile", "pdt_autotune", "pdt_set_tiles", "pdt_batch_solve",
+    "pdt_solver_destroy", "pdt_solver_profile", "pdt_batch_profile", "pdt_autotune", "pdt_set_tiles", "pdt_batch_solve", "pdt_solve_concurrent",
     "pdt_cnn_create", "pdt_cnn_destroy", "pdt_cnn_num_params", "pdt_cnn_set_params", "pdt_cnn_get_params",
     "pdt_cnn_forward", "pdt_cnn_loss_grad", "pdt_cnn_adam_step", "pdt_cnn_train_step", "pdt_solver_warm_start",
 ]
@@ -121,6 +121,7 @@ def lib():
         L.pdt_autotune.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ct.POINTER(PdtTiles), dp]
         L.pdt_set_tiles.argtypes = [ct.POINTER(vp), ct.c_int, ct.POINTER(PdtTiles)]
         L.pdt_batch_solve.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int]
+        L.pdt_solve_concurrent.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ct.c_int]
         fp, szp = ct.POINTER(ct.c_float), ct.POINTER(ct.c_size_t)
         L.pdt_cnn_create.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(vp)]
         L.pdt_cnn_destroy.argtypes = [vp]

@@ -315,6 +315,21 @@ def batch_solve(optimizers, verbose: bool = False) -> None:
     C.check(C.lib().pdt_batch_solve(arr, len(optimizers), int(bool(verbose))))
 
 
+def solve_concurrent(optimizers, streams: int = 4, verbose: bool = False) -> None:
+    """Solves configured optimizers each alone (as ``solve()`` would),
+    ``streams`` of them at a time: native worker threads take the next
+    optimizer in order and run its solve on its own HIP stream, so the
+    launches of different slices overlap on the GPU.  The optimizers may
+    differ in class, size and parameters; each one's result, stop and
+    statistics are those of its own ``solve()``."""
+    if not optimizers:
+        return
+    for o in optimizers:
+        o._push_params()
+    arr = (ct.c_void_p * len(optimizers))(*[o._h.value for o in optimizers])
+    C.check(C.lib().pdt_solve_concurrent(arr, len(optimizers), int(streams), int(bool(verbose))))
+
+
 # pyDenoising.cpp:63-93
 class TvOptimizer_2f(_Optimizer):
     _kind, _params_cls, _img_dtype = C.PDT_SOLVER_TV, TvParameters, np.float32

@@ -5,4 +5,4 @@ from ._core import (MriCartesianOperator, MriCartesianRemoveROOSOperator,  # noq
                     MriSamplingOperator, TgvMriOptimizer_2c2c, TgvMriOptimizer_2c3c,
                     TgvMriParameters, TgvOptimizer_2c2c, TgvOptimizer_2c3c, TgvParameters,
                     TvOptimizer_2c2c, TvOptimizerWithOp_2c3c, TvParameters, autotune, batch_profile, batch_solve,
-                    set_tiles)
+                    set_tiles, solve_concurrent)
