@@ -208,6 +208,31 @@ def test_batch_checks_fire_before_any_launch():
             L.pdt_solver_destroy(h)
 
 
+def test_concurrent_solve_checks_fire_before_any_launch():
+    """pdt_solve_concurrent refuses null or repeated solvers and a stream
+    count outside [1, PDT_MAX_CONCURRENT = 16] on the host, before any solve
+    starts (no GPU needed here).  It takes solvers of different kinds and
+    dtypes, which one batched engine would refuse."""
+    from primaldualtoolbox import _capi as C
+    L = C.lib()
+    a = _solver(C.PDT_SOLVER_TGV, C.PDT_C64, 0)
+    b = _solver(C.PDT_SOLVER_TV, C.PDT_F32)
+    try:
+        arr = lambda *hs: (ct.c_void_p * len(hs))(*hs)  # noqa: E731
+        cases = [(arr(a, b, a), 4, "appears twice"), (arr(a, None), 2, "null solver"),
+                 (arr(a, b), 0, "streams"), (arr(a, b), 17, "streams")]
+        for hs, streams, msg in cases:
+            assert L.pdt_solve_concurrent(hs, len(hs), streams, 0) == C.PDT_ERR_ARG
+            assert msg.encode() in L.pdt_last_error()
+        assert L.pdt_solve_concurrent(arr(a), 0, 4, 0) == C.PDT_ERR_ARG
+        assert L.pdt_batch_solve(arr(a, b), 2, 0) == C.PDT_ERR_ARG  # one engine: same kind and dtype
+        import primaldualtoolbox as pdt
+        assert pdt.mri.solve_concurrent is pdt.denoising.solve_concurrent
+    finally:
+        for h in (a, b):
+            L.pdt_solver_destroy(h)
+
+
 def test_params_before_any_state_are_free_to_change():
     """Before the first setInput0/solve no state exists, so dual_fp16 may
     still change (the GPU tests check the refusal once p, q exist)."""
