@@ -401,6 +401,33 @@ def extra_demo384(steps: int, cpu_threads: int, no_cpu: bool, autotune: bool = T
     return out
 
 
+def batch_fixed_iterations(count: int, workers: int, iters: int) -> None:
+    """Profiling mode: the batched slices of extra_batch64 (same seeds and
+    parameters) run `iters` iterations as one batched solve with direct
+    launches and no gap evaluation, so a hardware-counter run sees each
+    launch of a full-batch iteration (tiles from the tile cache or the
+    defaults).  Prints log lines only."""
+    import primaldualtoolbox as pdt
+    from primaldualtoolbox import synthetic as syn
+    cases = syn.batch_cases(range(1000, 1000 + count), workers=workers)
+    keep, solvers = [], []
+    for case in cases:
+        op = pdt.mri.MriCartesianRemoveROOSOperator()
+        op.setCoilSens(case["sens"])
+        op.setMask(case["mask"])
+        opt = pdt.mri.TgvMriOptimizer_2c3c()
+        opt.setOperator(op)
+        opt.setNoisyData(case["kspace"])
+        a1 = 1e-5 * float(np.linalg.norm(case["kspace"]))
+        prm = opt.getParameters()
+        prm.alpha1, prm.alpha0, prm.reduction, prm.max_iter = a1, 2 * a1, 1.0, iters - 1
+        prm.gap_every, prm.use_graph = 0, 0
+        keep.append(op)
+        solvers.append(opt)
+    pdt.mri.batch_solve(solvers)
+    log(f"[bench] batch: {count} slices, {iters} iterations with direct launches")
+
+
 def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool, autotune: bool = True,
                   graph: bool = True, runs: int = 3) -> dict:
     """`count` mixed Cartesian TGV-MRI slices (sizes 256^2..384x320, 8-20
@@ -580,6 +607,9 @@ def main():
                     help="profiling mode: run this many iterations per step (no gap stop), print no JSON")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the tile cache (PDT_TUNE_CACHE) or defaults instead of timing candidates")
+    ap.add_argument("--batch-fixed-iters", type=int, default=0,
+                    help="profiling mode: the --batch slices run this many iterations as one batched solve "
+                         "with direct launches (no gap evaluation), print no JSON")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="profiling mode: run a plain device copy of this many MiB (PMC byte calibration)")
     ap.add_argument("--no-extra", action="store_true",
@@ -611,6 +641,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         log(f"[bench] copied {args.calibrate_copy} MiB x 5")
+        return
+
+    if args.batch_fixed_iters:
+        batch_fixed_iterations(args.batch, args.gen_workers, args.batch_fixed_iters)
         return
 
     if args.only:
