@@ -271,10 +271,12 @@ PDT_API int pdt_batch_solve(pdt_solver** solvers, int count, int verbose);
  * worker threads (the calling thread among them) take the next unsolved
  * solver in the given order and run its solve on the solver's own HIP
  * stream, so the launches of different slices overlap on the GPU.  Solvers
- * may differ in kind, dtype, size and parameters; each one's iterates, stop
- * and statistics are those of its own pdt_solver_solve.  Null or repeated
- * solvers are refused before any solve starts; after a failure no further
- * solver is started and the first error is returned. */
+ * may differ in kind, dtype, size and parameters, and may share an
+ * operator; each one's iterates, stop and statistics are those of its own
+ * pdt_solver_solve (its device time is the elapsed time of its stream,
+ * which the slices running beside it share).  Null or repeated solvers are
+ * refused before any solve starts; after a failure no further solver is
+ * started and the first error is returned. */
 #define PDT_MAX_CONCURRENT 16
 PDT_API int pdt_solve_concurrent(pdt_solver** solvers, int count, int streams, int verbose);
 
