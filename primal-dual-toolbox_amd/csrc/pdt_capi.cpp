@@ -1649,10 +1649,12 @@ void solve_many(std::vector<pdt_solver*>& sv, int verbose) {
     cached->run(verbose);
     return;
   }
-  Engine e;
-  e.sv = sv;
-  e.build();
-  e.run(verbose);
+  auto e = std::make_unique<Engine>();
+  e->sv = sv;
+  e->build();
+  e->run(verbose);
+  SetupLock lk(g_setup_mu);  // its graphs and device buffers are released under the lock
+  e.reset();
 }
 
 }  // namespace
@@ -1927,7 +1929,10 @@ int pdt_op_name(const pdt_op* op, char* buf, size_t len) {
 }
 
 int pdt_op_destroy(pdt_op* op) {
-  return guard([&] { op_release(op); });
+  return guard([&] {
+    SetupLock lk(g_setup_mu);
+    op_release(op);
+  });
 }
 
 // ---- solvers ----
@@ -2043,7 +2048,10 @@ int pdt_solver_get_stats(const pdt_solver* s, pdt_stats* st) {
 }
 
 int pdt_solver_destroy(pdt_solver* s) {
-  return guard([&] { delete s; });
+  return guard([&] {
+    SetupLock lk(g_setup_mu);  // its engine's graphs and its device buffers
+    delete s;
+  });
 }
 
 // The solvers' iterates (u, ubar, v, vbar, p, q and the MRI dual r), copied
