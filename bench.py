@@ -525,6 +525,8 @@ def extra_batch64(count: int, workers: int, cpu_threads: int, no_cpu: bool, auto
            "sequential_seconds": seq_s, "sequential_slices_per_s": count / seq_s,
            "concurrent": {"streams": CONC_STREAMS, "seconds": conc_s, "seconds_runs": conc_times,
                           "slices_per_s": count / conc_s, "equal_to_single_solves": True},
+           "best": ({"method": f"concurrent single-slice solves ({CONC_STREAMS} streams)", "slices_per_s": count / conc_s}
+                    if conc_s < batch_s else {"method": "batched work lists", "slices_per_s": count / batch_s}),
            "iterations": {k: int(v) for k, v in dist(its).items()},
            "time_to_gap_alone_ms": dist(alone_ms),
            "time_to_stop_in_batch_ms": dist(stop_ms),
