@@ -27,6 +27,7 @@ from . import synthetic as syn
 from . import warmstart as ws
 
 GAP_TOL, GAP_EVERY = 1e-4, 50
+SOLVE_STREAMS = 4  # slices solved at once (pdt_solve_concurrent)
 
 
 def log(*a):
@@ -48,9 +49,10 @@ def build_solver(case, max_iter: int = 200000):
 
 
 def solve_pairs(cases):
-    """Solves the slices together and returns (x, t, iterations)."""
+    """Solves the slices (several at a time, each as its own solve) and
+    returns (x, t, iterations)."""
     built = [build_solver(c) for c in cases]
-    mri.batch_solve([o for _, o in built])
+    mri.solve_concurrent([o for _, o in built], streams=SOLVE_STREAMS)
     xs, ts, its = [], [], []
     for (op, opt), c in zip(built, cases):
         st = opt.getStats()
@@ -138,7 +140,7 @@ def train(net, X, T, Xv, Tv, epochs: int, batch: int, lr: float, seed: int, lr_f
 
 def warm_vs_cold(net, seeds, workers: int, chunk: int = 64):
     """Per held-out slice: iterations to the gap from the cold start and from
-    the network's (u, v), both in batched solves."""
+    the network's (u, v), several slices at a time."""
     cold, warm = [], []
     with syn.process_pool(workers) as ex:
         for k in range(0, len(seeds), chunk):
@@ -148,7 +150,7 @@ def warm_vs_cold(net, seeds, workers: int, chunk: int = 64):
                 if use_net:
                     for _, o in built:
                         net.warm_start(o)
-                mri.batch_solve([o for _, o in built])
+                mri.solve_concurrent([o for _, o in built], streams=SOLVE_STREAMS)
                 its = [int(o.getStats()["iters"]) for _, o in built]
                 (warm if use_net else cold).extend(its)
             log(f"[train] held-out {k + len(cases)}/{len(seeds)}: median iterations cold {int(np.median(cold))} "
